@@ -1,0 +1,352 @@
+"""ctypes binding of the oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the FP64 CPU restatement of the reference path
+that the GPU library is checked against.  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "tensor-tools_amd", "python"))
+from ttgpu.flat import FlatTensor, Index  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_i = C.c_int
+_d = C.c_double
+_l = C.c_long
+_p = C.c_void_p
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_tensor_import.restype = _p
+        L.orc_tensor_import.argtypes = [_i, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip, _i, _ip, _dp]
+        L.orc_init_block.restype = _p
+        L.orc_init_block.argtypes = [_i, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip]
+        L.orc_tensor_info.argtypes = [_p, _ip, C.POINTER(_l)]
+        L.orc_tensor_export.argtypes = [_p, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip, _ip, _lp, _dp]
+        L.orc_tensor_free.argtypes = [_p]
+        L.orc_tensor_copy.restype = _p
+        L.orc_tensor_copy.argtypes = [_p]
+        L.orc_contract.restype = _p
+        L.orc_contract.argtypes = [_p, _p]
+        L.orc_last_tasks.restype = _i
+        L.orc_last_tasks.argtypes = [_ip, _i]
+        L.orc_pairs.restype = _i
+        L.orc_pairs.argtypes = [_p, _p, _ip, _ip]
+        for f in ("orc_inner",):
+            getattr(L, f).restype = _d
+            getattr(L, f).argtypes = [_p, _p]
+        L.orc_norm.restype = _d
+        L.orc_norm.argtypes = [_p]
+        L.orc_add.argtypes = [_p, _p, _d]
+        L.orc_scale.argtypes = [_p, _d]
+        L.orc_prime.argtypes = [_p, _i]
+        L.orc_dag.argtypes = [_p]
+        L.orc_big_new.restype = _p
+        L.orc_big_new.argtypes = [_p, _p, _p, _p]
+        L.orc_big_free.argtypes = [_p]
+        L.orc_big_mid.restype = _p
+        L.orc_big_mid.argtypes = [_p]
+        L.orc_big_product.restype = _p
+        L.orc_big_product.argtypes = [_p, _p]
+        L.orc_big_expec.restype = _d
+        L.orc_big_expec.argtypes = [_p, _p]
+        L.orc_big_size.restype = _l
+        L.orc_big_size.argtypes = [_p]
+        L.orc_davidson.restype = _d
+        L.orc_davidson.argtypes = [_p, _p, _i, _d, C.POINTER(_i)]
+        L.orc_heff_time.restype = _d
+        L.orc_heff_time.argtypes = [_p, _p, _i]
+        L.orc_threads.restype = _i
+        L.orc_env_step.restype = _p
+        L.orc_env_step.argtypes = [_p, _p, _p]
+        L.orc_svd_bond.restype = _d
+        L.orc_svd_bond.argtypes = [_p, _p, _p, C.c_char_p, _i, _d, _l, C.POINTER(_i), C.POINTER(_p)]
+        L.orc_dense_svd.argtypes = [_i, _i, _dp, _dp, _dp, _dp]
+        L.orc_diag.argtypes = [_i, _dp, _dp]
+        L.orc_mpo_heisenberg_nn.restype = _p
+        L.orc_mpo_heisenberg_nn.argtypes = [_i, _i]
+        L.orc_mpo_autompo.restype = _p
+        L.orc_mpo_autompo.argtypes = [_i, _i, _i, _dp, _ip, _ip, _ip, _ip]
+        L.orc_heisenberg_terms.restype = _i
+        L.orc_heisenberg_terms.argtypes = [_i, _i, _i, _i, _d, _d, _i, _dp, _ip, _ip, _ip, _ip]
+        L.orc_mps_product.restype = _p
+        L.orc_mps_product.argtypes = [_i, _i, _ip]
+        L.orc_qtt_tensor.restype = _p
+        L.orc_qtt_tensor.argtypes = [_p, _i]
+        L.orc_qtt_set_tensor.argtypes = [_p, _i, _p]
+        L.orc_qtt_length.restype = _i
+        L.orc_qtt_length.argtypes = [_p]
+        L.orc_qtt_center.restype = _i
+        L.orc_qtt_center.argtypes = [_p]
+        L.orc_qtt_free.argtypes = [_p]
+        L.orc_qtt_position.restype = _d
+        L.orc_qtt_position.argtypes = [_p, _i]
+        L.orc_qtt_normalize.argtypes = [_p]
+        L.orc_qtt_bond_dims.argtypes = [_p, _ip]
+        L.orc_dmrg.restype = _d
+        L.orc_dmrg.argtypes = [_p, _p, _i, _i, _ip, _i, _dp, _i, _ip, _d, _dp, C.POINTER(_d),
+                               C.POINTER(_d)]
+        L.orc_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(h):
+    if not h:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+    return h
+
+
+class OTensor:
+    """Owning handle to an oracle qtensor."""
+
+    def __init__(self, h):
+        self.h = _check(h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_tensor_free(self.h)
+            self.h = None
+
+    @classmethod
+    def from_flat(cls, t: FlatTensor) -> "OTensor":
+        a, lv, ty, nm, ns, qn, qd, bqi, data = t.c_arrays()
+        if len(data) == 0:
+            data = np.zeros(1)
+        if len(bqi) == 0:
+            bqi = np.zeros(1, dtype=np.int32)
+        return cls(lib().orc_tensor_import(t.rank, a, lv, ty, nm, ns, qn, qd, t.nblocks, bqi, data))
+
+    def to_flat(self) -> FlatTensor:
+        info = np.zeros(4, dtype=np.int32)
+        tot = _l(0)
+        lib().orc_tensor_info(self.h, info, C.byref(tot))
+        rank, nst, nb, nl = (int(v) for v in info)
+        arrows = np.zeros(max(rank, 1), np.int32)
+        levels = np.zeros(max(rank, 1), np.int32)
+        types = np.zeros(max(rank, 1), np.int32)
+        names = C.create_string_buffer(nl + 1)
+        nsec = np.zeros(max(rank, 1), np.int32)
+        qn = np.zeros(max(2 * nst, 1), np.int32)
+        qd = np.zeros(max(nst, 1), np.int32)
+        bqi = np.zeros(max(nb * rank, 1), np.int32)
+        boff = np.zeros(nb + 1, np.int64)
+        data = np.zeros(max(tot.value, 1))
+        lib().orc_tensor_export(self.h, arrows, levels, types, names, nsec, qn, qd, bqi, boff, data)
+        nmlist = names.value.decode().split("\n")
+        idx, off = [], 0
+        for i in range(rank):
+            k = int(nsec[i])
+            idx.append(Index(int(arrows[i]), nmlist[i], int(types[i]), int(levels[i]),
+                             [(int(qn[2 * (off + s)]), int(qn[2 * (off + s) + 1])) for s in range(k)],
+                             [int(qd[off + s]) for s in range(k)]))
+            off += k
+        return FlatTensor(idx, bqi[:nb * rank].reshape(nb, rank).copy(), data[:tot.value].copy())
+
+    def copy(self):
+        return OTensor(lib().orc_tensor_copy(self.h))
+
+    def __mul__(self, other):
+        return OTensor(lib().orc_contract(self.h, other.h))
+
+    def pairs(self, other):
+        pa = np.zeros(16, np.int32)
+        pb = np.zeros(16, np.int32)
+        n = lib().orc_pairs(self.h, other.h, pa, pb)
+        return list(zip(pa[:n].tolist(), pb[:n].tolist()))
+
+    def inner(self, other):
+        return lib().orc_inner(self.h, other.h)
+
+    def norm(self):
+        return lib().orc_norm(self.h)
+
+    def add(self, other, c=1.0):
+        lib().orc_add(self.h, other.h, c)
+
+    def scale(self, c):
+        lib().orc_scale(self.h, c)
+
+    def prime(self, inc=1):
+        lib().orc_prime(self.h, inc)
+
+    def dag(self):
+        lib().orc_dag(self.h)
+
+
+def last_tasks() -> np.ndarray:
+    """(c, a, b, M, N, K) rows of the last oracle contraction, loop order."""
+    n = lib().orc_last_tasks(np.zeros(6, np.int32), 0)
+    out = np.zeros(6 * max(n, 1), np.int32)
+    lib().orc_last_tasks(out, n)
+    return out[:6 * n].reshape(n, 6)
+
+
+def init_block(idx):
+    t = FlatTensor(idx, np.zeros((0, len(idx)), np.int32), np.zeros(0))
+    a, lv, ty, nm, ns, qn, qd, _, _ = t.c_arrays()
+    return OTensor(lib().orc_init_block(len(idx), a, lv, ty, nm, ns, qn, qd))
+
+
+class OBig:
+    """Oracle big_qtensor: L * (W1 W2) * R."""
+
+    def __init__(self, L: OTensor, W1: OTensor, W2, R: OTensor):
+        self._keep = (L, W1, W2, R)
+        self.h = lib().orc_big_new(L.h, W1.h if W1 else None, W2.h if W2 else None, R.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_big_free(self.h)
+
+    def product(self, x: OTensor) -> OTensor:
+        return OTensor(lib().orc_big_product(self.h, x.h))
+
+    def mid(self) -> OTensor:
+        return OTensor(lib().orc_big_mid(self.h))
+
+    def expec(self, x):
+        return lib().orc_big_expec(self.h, x.h)
+
+    def size(self):
+        return lib().orc_big_size(self.h)
+
+    def davidson(self, x: OTensor, max_restart=2, tol=1e-12):
+        n = _i(0)
+        e = lib().orc_davidson(self.h, x.h, max_restart, tol, C.byref(n))
+        return e, n.value
+
+    def time_products(self, x: OTensor, n: int) -> float:
+        return lib().orc_heff_time(self.h, x.h, n)
+
+
+def env_step(psi: OTensor, T: OTensor, W: OTensor) -> OTensor:
+    return OTensor(lib().orc_env_step(psi.h, T.h, W.h))
+
+
+def svd_bond(combined: OTensor, A_left: OTensor, A_right: OTensor, mid_name: str, direction: int,
+             cutoff: float, K: int):
+    kept = _i(0)
+    S = _p()
+    tr = lib().orc_svd_bond(combined.h, A_left.h, A_right.h, mid_name.encode(), direction, cutoff,
+                            K, C.byref(kept), C.byref(S))
+    if tr < 0:
+        raise RuntimeError("oracle svd_bond: " + lib().orc_last_error().decode())
+    return tr, kept.value, OTensor(S.value)
+
+
+def dense_svd(A: np.ndarray):
+    A = np.asfortranarray(A, dtype=np.float64)
+    r, c = A.shape
+    k = min(r, c)
+    U = np.zeros(r * k)
+    S = np.zeros(k)
+    Vt = np.zeros(k * c)
+    lib().orc_dense_svd(r, c, np.ascontiguousarray(A.reshape(-1, order="F")), U, S, Vt)
+    return U.reshape((r, k), order="F"), S, Vt.reshape((k, c), order="F")
+
+
+def diag(M: np.ndarray):
+    n = M.shape[0]
+    a = np.ascontiguousarray(np.asfortranarray(M).reshape(-1, order="F"))
+    ev = np.zeros(n)
+    lib().orc_diag(n, a, ev)
+    return ev, a.reshape((n, n), order="F")
+
+
+OPS = {"Sz": 0, "S+": 1, "S-": 2}
+
+
+class OQTT:
+    def __init__(self, h):
+        self.h = _check(h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_qtt_free(self.h)
+
+    def __len__(self):
+        return lib().orc_qtt_length(self.h)
+
+    def tensor(self, i) -> OTensor:
+        return OTensor(lib().orc_qtt_tensor(self.h, i))
+
+    def set_tensor(self, i, t: OTensor):
+        lib().orc_qtt_set_tensor(self.h, i, t.h)
+
+    def position(self, site):
+        return lib().orc_qtt_position(self.h, site)
+
+    def normalize(self):
+        lib().orc_qtt_normalize(self.h)
+
+    def center(self):
+        return lib().orc_qtt_center(self.h)
+
+    def bond_dims(self):
+        out = np.zeros(len(self) + 1, np.int32)
+        lib().orc_qtt_bond_dims(self.h, out)
+        return out
+
+
+def heisenberg_terms(kind: int, Nx: int, Ny: int, yper: bool, J1=1.0, J2=0.0):
+    cap = 6 * 4 * Nx * Ny + 16
+    coef = np.zeros(cap)
+    o1, i1, o2, i2 = (np.zeros(cap, np.int32) for _ in range(4))
+    n = lib().orc_heisenberg_terms(kind, Nx, Ny, int(yper), J1, J2, cap, coef, o1, i1, o2, i2)
+    return coef[:n].copy(), o1[:n].copy(), i1[:n].copy(), o2[:n].copy(), i2[:n].copy()
+
+
+def mpo_autompo(N: int, terms, mpo_id: int = 2) -> OQTT:
+    coef, o1, i1, o2, i2 = terms
+    return OQTT(lib().orc_mpo_autompo(N, mpo_id, len(coef), np.ascontiguousarray(coef), o1, i1, o2, i2))
+
+
+def mpo_heisenberg_nn(N: int, mpo_id: int = 2) -> OQTT:
+    return OQTT(lib().orc_mpo_heisenberg_nn(N, mpo_id))
+
+
+def mps_product(state, mps_id: int = 1) -> OQTT:
+    st = np.ascontiguousarray(np.array(state, dtype=np.int32))
+    return OQTT(lib().orc_mps_product(len(st), mps_id, st))
+
+
+def dmrg(psi: OQTT, H: OQTT, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12):
+    mb = np.ascontiguousarray(np.array(maxbd, np.int32))
+    ct = np.ascontiguousarray(np.array(cutoff, np.float64))
+    mr = np.ascontiguousarray(np.array(max_restart, np.int32))
+    hs = np.zeros(2 * nsweeps)
+    e0 = _d(0)
+    sec = _d(0)
+    E = lib().orc_dmrg(psi.h, H.h, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr, tol, hs,
+                       C.byref(e0), C.byref(sec))
+    return E, hs, e0.value, sec.value
+
+
+def threads() -> int:
+    return lib().orc_threads()

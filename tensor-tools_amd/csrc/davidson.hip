@@ -1,0 +1,147 @@
+// davidson.hip -- device pieces of tensor_davidsonIT (tensor_davidson.cpp:192-347)
+// that are not BLAS-1: the FP64 Rayleigh-Ritz eigensolver of the small
+// Krylov matrix (DIAG/dsyev, lapack_wrapper.cpp:40-54) and block-list
+// embedding (qtensor::add appends missing blocks, qtensor.cpp:1127-1156).
+#include <algorithm>
+
+#include "ttg_device.h"
+#include "ttg_internal.h"
+
+namespace ttg {
+
+// Cyclic Jacobi (Golub & Van Loan 8.5) on an n x n symmetric matrix, one
+// thread (n <= 16).  Values ascending, vectors in columns (column-major).
+__global__ void k_small_eigh(int n, const double* __restrict__ Min, double* __restrict__ evals,
+                             double* __restrict__ evecs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a[16 * 16], v[16 * 16];
+  for (int i = 0; i < n * n; i++) a[i] = Min[i];
+  for (int i = 0; i < n * n; i++) v[i] = 0.0;
+  for (int i = 0; i < n; i++) v[i + n * i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n * n; i++) tot += a[i] * a[i];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++)
+        if (i != j) off += a[i + n * j] * a[i + n * j];
+    if (off == 0.0 || off <= 1e-32 * tot) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = a[p + n * q];
+        if (apq == 0.0) continue;
+        double theta = (a[q + n * q] - a[p + n * p]) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {
+          double x = a[k + n * p], y = a[k + n * q];
+          a[k + n * p] = c * x - s * y;
+          a[k + n * q] = s * x + c * y;
+        }
+        for (int k = 0; k < n; k++) {
+          double x = a[p + n * k], y = a[q + n * k];
+          a[p + n * k] = c * x - s * y;
+          a[q + n * k] = s * x + c * y;
+        }
+        for (int k = 0; k < n; k++) {
+          double x = v[k + n * p], y = v[k + n * q];
+          v[k + n * p] = c * x - s * y;
+          v[k + n * q] = s * x + c * y;
+        }
+      }
+  }
+  // stable selection sort by eigenvalue
+  int ord[16];
+  for (int i = 0; i < n; i++) ord[i] = i;
+  for (int i = 1; i < n; i++) {
+    int o = ord[i];
+    int j = i - 1;
+    while (j >= 0 && a[ord[j] + n * ord[j]] > a[o + n * o]) {
+      ord[j + 1] = ord[j];
+      j--;
+    }
+    ord[j + 1] = o;
+  }
+  for (int j = 0; j < n; j++) {
+    evals[j] = a[ord[j] + n * ord[j]];
+    for (int i = 0; i < n; i++) evecs[i + n * j] = v[i + n * ord[j]];
+  }
+}
+
+// dst (zeroed) receives every block of src at the same sector tuple
+__global__ void k_embed_map(MetaView src, const uint64_t* dkeys, const int* dids, int nd,
+                            const int64_t* doff, int* map, int* err) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= src.nblocks) return;
+  uint64_t k = 0;
+  for (int j = 0; j < src.rank; j++) k = k * src.nsec[j] + src.qi[(size_t)b * src.rank + j];
+  int p = lower_bound_u64(dkeys, nd, k);
+  if (p >= nd || dkeys[p] != k) {
+    atomicAdd(err, 1);
+    map[b] = -1;
+    return;
+  }
+  map[b] = dids[p];
+}
+
+__global__ void k_embed_copy(int nb, const int* map, const int64_t* soff, const int64_t* doff,
+                             const double* __restrict__ s, double* __restrict__ d) {
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int db = map[b];
+    if (db < 0) continue;
+    const int64_t n = soff[b + 1] - soff[b];
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) d[doff[db] + i] = s[soff[b] + i];
+  }
+}
+
+}  // namespace ttg
+
+using namespace ttg;
+
+extern "C" {
+
+int ttg_small_eigh(int n, const double* M, double* evals, double* evecs) {
+  TTG_TRY(ensure_init());
+  TTG_REQUIRE(n >= 1 && n <= 16, "small_eigh: 1 <= n <= 16");
+  double* d = dalloc<double>(n * n * 2 + n);
+  TTG_HIP(hipMemcpyAsync(d, M, sizeof(double) * n * n, hipMemcpyHostToDevice, stream()));
+  hipLaunchKernelGGL(k_small_eigh, dim3(1), dim3(64), 0, stream(), n, d, d + 2 * n * n, d + n * n);
+  TTG_HIP(hipMemcpyAsync(evecs, d + n * n, sizeof(double) * n * n, hipMemcpyDeviceToHost, stream()));
+  TTG_HIP(hipMemcpyAsync(evals, d + 2 * n * n, sizeof(double) * n, hipMemcpyDeviceToHost, stream()));
+  TTG_HIP(hipStreamSynchronize(stream()));
+  dfree(d);
+  return 0;
+}
+
+int ttg_embed(ttg_tensor src, ttg_tensor dst) {
+  TTG_REQUIRE(src && dst, "embed: null tensor");
+  TTG_REQUIRE(src->m->same_indices(*dst->m), "embed: index sectors differ");
+  hipStream_t s = stream();
+  const int ns = src->m->nblocks, nd = dst->m->nblocks;
+  uint64_t* keys = dalloc<uint64_t>(std::max(nd, 1));
+  uint64_t* skeys = dalloc<uint64_t>(std::max(nd, 1));
+  int* ids = dalloc<int>(std::max(nd, 1));
+  int* sids = dalloc<int>(std::max(nd, 1));
+  int* map = dalloc<int>(std::max(ns, 1));
+  int* err = dalloc<int>(1);
+  int herr = 0;
+  TTG_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  fill_f64(dst->d_data, dst->m->total, 0.0);
+  if (nd) {
+    hipLaunchKernelGGL(k_tuple_keys, dim3((nd + 255) / 256), dim3(256), 0, s, dst->m->view(), keys, ids);
+    hipLaunchKernelGGL(k_sort_keys, dim3((nd + 255) / 256), dim3(256), 0, s, keys, ids, nd, skeys, sids);
+  }
+  if (ns) {
+    hipLaunchKernelGGL(k_embed_map, dim3((ns + 255) / 256), dim3(256), 0, s, src->m->view(), skeys,
+                       sids, nd, dst->m->d_off, map, err);
+    hipLaunchKernelGGL(k_embed_copy, dim3(std::min(ns, 2048)), dim3(256), 0, s, ns, map,
+                       src->m->d_off, dst->m->d_off, src->d_data, dst->d_data);
+  }
+  TTG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  dfree(keys); dfree(skeys); dfree(ids); dfree(sids); dfree(map); dfree(err);
+  mark_written(dst);
+  TTG_REQUIRE(herr == 0, "embed: source block missing from destination block list");
+  return 0;
+}
+
+}  // extern "C"

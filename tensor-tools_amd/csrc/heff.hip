@@ -1,0 +1,245 @@
+// heff.hip -- lazy effective Hamiltonian y = prime(-1)[((L*x)*W12)*R]
+// (big_qtensor::product, big_qtensor.cpp:23-40) as three planned grouped
+// GEMM stages replayed from one HIP graph.
+//
+// The three stage plans are built once per operator (the block structure of
+// L, W12, R and x does not change inside a Davidson call).  Stage outputs
+// T1 and T2 are materialised in HBM: FP64 for TTG_FP64, FP32 for the
+// reduced-precision forms (whose operands are FP32 shadows of L, W12 and R
+// made once per operator).  The last stage writes y in FP64 directly into
+// x's block layout: every output block's offset is remapped to the offset
+// of the same sector tuple in x, so y and x share one block list and the
+// Davidson vector ops stay plain packed-array BLAS-1.
+#include <algorithm>
+
+#include "ttg_device.h"
+#include "ttg_internal.h"
+#include "ttg_plan.h"
+
+struct ttg_heff_s {
+  ttg_plan p[3] = {nullptr, nullptr, nullptr};
+  ttg::MetaPtr xm;               // x / y block structure
+  int prec = TTG_FP64;
+  ttg_tensor L = nullptr, W = nullptr, R = nullptr;   // not owned (reference: raw pointers)
+  void* T1 = nullptr;
+  void* T2 = nullptr;
+  float *L32 = nullptr, *W32 = nullptr, *R32 = nullptr, *x32 = nullptr;
+  double* xbuf = nullptr;        // graph-bound input / output buffers
+  double* ybuf = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  double flops = 0;
+  ~ttg_heff_s() {
+    for (auto* q : p) delete q;
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    ttg::dfree(T1); ttg::dfree(T2);
+    ttg::dfree(L32); ttg::dfree(W32); ttg::dfree(R32); ttg::dfree(x32);
+    ttg::dfree(xbuf); ttg::dfree(ybuf);
+  }
+};
+
+namespace ttg {
+
+__global__ void k_cvt_f32(const double* __restrict__ s, float* __restrict__ d, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    d[i] = (float)s[i];
+}
+
+static unsigned grid_n(size_t n) {
+  size_t g = (n + 255) / 256;
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 4096));
+}
+
+// point every output block of the last stage at the same tuple's offset in x
+__global__ void k_remap(MetaView c, const uint64_t* xkeys, const int* xids, int nx,
+                        const int64_t* xoff, const int* xqi_dims_dummy, CBlock* cblk, int* err) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= c.nblocks) return;
+  uint64_t k = 0;
+  for (int j = 0; j < c.rank; j++) k = k * c.nsec[j] + c.qi[(size_t)b * c.rank + j];
+  int p = lower_bound_u64(xkeys, nx, k);
+  if (p >= nx || xkeys[p] != k) {
+    atomicAdd(err, 1);
+    return;
+  }
+  int xb = xids[p];
+  if (xoff[xb + 1] - xoff[xb] != (int64_t)cblk[b].M * cblk[b].N) {
+    atomicAdd(err, 1);
+    return;
+  }
+  cblk[b].c_off = xoff[xb];
+}
+
+static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
+  hipStream_t s = stream();
+  const int nx = xm->nblocks, nc = p3->C->nblocks;
+  uint64_t* keys = dalloc<uint64_t>(std::max(nx, 1));
+  uint64_t* skeys = dalloc<uint64_t>(std::max(nx, 1));
+  int* ids = dalloc<int>(std::max(nx, 1));
+  int* sids = dalloc<int>(std::max(nx, 1));
+  int* err = dalloc<int>(1);
+  int herr = 0;
+  TTG_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  if (nx) {
+    hipLaunchKernelGGL(k_tuple_keys, dim3((nx + 255) / 256), dim3(256), 0, s, xm->view(), keys, ids);
+    hipLaunchKernelGGL(k_sort_keys, dim3((nx + 255) / 256), dim3(256), 0, s, keys, ids, nx, skeys, sids);
+  }
+  if (nc)
+    hipLaunchKernelGGL(k_remap, dim3((nc + 255) / 256), dim3(256), 0, s, p3->C->view(), skeys, sids,
+                       nx, xm->d_off, (const int*)nullptr, p3->d_cblk, err);
+  TTG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  dfree(keys); dfree(skeys); dfree(ids); dfree(sids); dfree(err);
+  TTG_REQUIRE(herr == 0, "heff: the product has blocks outside x's block list "
+                         "(give x the full legal block list, ttg_tensor_init_blocks)");
+  return 0;
+}
+
+static int record(ttg_heff h, hipStream_t s) {
+  const bool red = h->prec != TTG_FP64;
+  const size_t nx = h->xm->total;
+  if (red) {
+    hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(nx)), dim3(256), 0, s, h->xbuf, h->x32, nx);
+    TTG_TRY(gemm_launch(*h->p[0], h->L32, h->x32, h->T1, false, s));
+    TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W32, h->T2, false, s));
+    TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R32, h->ybuf, true, s));
+  } else {
+    TTG_TRY(gemm_launch(*h->p[0], h->L->d_data, h->xbuf, h->T1, true, s));
+    TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W->d_data, h->T2, true, s));
+    TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R->d_data, h->ybuf, true, s));
+  }
+  return 0;
+}
+
+static int to_f32(ttg_tensor t, float** out) {
+  *out = dalloc<float>(t->m->total);
+  TTG_REQUIRE(*out, "heff: allocation failed");
+  hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(t->m->total)), dim3(256), 0, stream(), t->d_data, *out,
+                     t->m->total);
+  return 0;
+}
+
+}  // namespace ttg
+
+using namespace ttg;
+
+extern "C" {
+
+int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_like,
+                    const int* pairs3, int prec, ttg_heff* out) {
+  TTG_REQUIRE(L && W12 && R && x_like && out, "heff_create: null argument");
+  TTG_REQUIRE(prec == TTG_FP64 || prec == TTG_FP32 || prec == TTG_BF16X3, "unknown precision");
+  ttg_heff h = new ttg_heff_s();
+  h->prec = prec;
+  h->L = L; h->W = W12; h->R = R;
+  h->xm = x_like->m;
+  const int* pp = pairs3;
+  MetaPtr lhs[3] = {L->m, nullptr, nullptr};
+  MetaPtr rhs[3] = {x_like->m, W12->m, R->m};
+  for (int s = 0; s < 3; s++) {
+    int np = *pp++;
+    std::vector<int> pa(np), pb(np);
+    for (int i = 0; i < np; i++) { pa[i] = *pp++; pb[i] = *pp++; }
+    MetaPtr a = s == 0 ? lhs[0] : h->p[s - 1]->C;
+    int r = plan_build(a, rhs[s], np, pa.data(), pb.data(), prec, -1, &h->p[s]);
+    if (r) { delete h; return r; }
+    h->flops += h->p[s]->flops;
+  }
+  // the last stage's result indices must be x's (after prime(-1))
+  if (!h->p[2]->C->same_indices(*h->xm)) {
+    delete h;
+    set_error("heff: L*x*W12*R does not reproduce x's index sectors");
+    return 1;
+  }
+  if (remap_to_x(h->p[2], h->xm)) { delete h; return 1; }
+  const bool red = prec != TTG_FP64;
+  const size_t es = red ? sizeof(float) : sizeof(double);
+  h->T1 = dmalloc(std::max<size_t>(1, h->p[0]->C->total) * es);
+  h->T2 = dmalloc(std::max<size_t>(1, h->p[1]->C->total) * es);
+  h->xbuf = dalloc<double>(h->xm->total);
+  h->ybuf = dalloc<double>(h->xm->total);
+  if (!h->T1 || !h->T2 || !h->xbuf || !h->ybuf) { delete h; set_error("heff: allocation failed"); return 1; }
+  fill_f64(h->ybuf, h->xm->total, 0.0);   // blocks of x never produced stay zero
+  if (red) {
+    h->x32 = dalloc<float>(h->xm->total);
+    if (to_f32(L, &h->L32) || to_f32(W12, &h->W32) || to_f32(R, &h->R32)) { delete h; return 1; }
+  }
+  // capture the apply sequence once
+  hipStream_t s = stream();
+  TTG_HIP(hipStreamSynchronize(s));
+  TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rr = record(h, s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rr || e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    delete h;
+    set_error("heff: graph capture failed");
+    return 1;
+  }
+  h->graph = g;
+  TTG_HIP(hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0));
+  *out = h;
+  return 0;
+}
+
+int ttg_heff_new_output(ttg_heff h, ttg_tensor* y) {
+  TTG_REQUIRE(h, "null heff");
+  return tensor_from_meta(h->xm, true, y);
+}
+
+int ttg_heff_apply(ttg_heff h, ttg_tensor x, ttg_tensor y) {
+  TTG_REQUIRE(h && x && y, "heff_apply: null argument");
+  TTG_REQUIRE(x->m->total == h->xm->total && y->m->total == h->xm->total,
+              "heff_apply: x/y do not have the operator's block list");
+  hipStream_t s = stream();
+  const size_t bytes = h->xm->total * sizeof(double);
+  TTG_HIP(hipMemcpyAsync(h->xbuf, x->d_data, bytes, hipMemcpyDeviceToDevice, s));
+  TTG_HIP(hipGraphLaunch(h->exec, s));
+  TTG_HIP(hipMemcpyAsync(y->d_data, h->ybuf, bytes, hipMemcpyDeviceToDevice, s));
+  mark_written(y);
+  return 0;
+}
+
+int ttg_heff_info(ttg_heff h, double* flops, int64_t* ntasks3, double* stage_flops3) {
+  TTG_REQUIRE(h, "null heff");
+  if (flops) *flops = h->flops;
+  for (int s = 0; s < 3; s++) {
+    if (ntasks3) ntasks3[s] = h->p[s]->ntasks;
+    if (stage_flops3) stage_flops3[s] = h->p[s]->flops;
+  }
+  return 0;
+}
+
+int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms) {
+  TTG_REQUIRE(h && n > 0, "heff_time: bad argument");
+  hipStream_t s = stream();
+  const size_t bytes = h->xm->total * sizeof(double);
+  TTG_HIP(hipMemcpyAsync(h->xbuf, x->d_data, bytes, hipMemcpyDeviceToDevice, s));
+  hipEvent_t e0, e1;
+  TTG_HIP(hipEventCreate(&e0));
+  TTG_HIP(hipEventCreate(&e1));
+  TTG_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < n; i++) TTG_HIP(hipGraphLaunch(h->exec, s));
+  TTG_HIP(hipEventRecord(e1, s));
+  TTG_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  TTG_HIP(hipEventElapsedTime(&t, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  TTG_HIP(hipMemcpyAsync(y->d_data, h->ybuf, bytes, hipMemcpyDeviceToDevice, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  mark_written(y);
+  *ms = t / n;
+  return 0;
+}
+
+int ttg_heff_free(ttg_heff h) {
+  if (h) hipStreamSynchronize(stream());
+  delete h;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,713 @@
+// plan.hip -- device QN-sector bookkeeping for C = A * B (qtensor.cpp:523-734).
+//
+// The reference builds, on the host, per mid-QN q the ordered sets of A's
+// free-index position tuples (left), B's free tuples (right) and B's
+// contracted tuples (mid), creates one result block per (q, right, left) in
+// (ascending q, lexicographic right, lexicographic left) order, and runs one
+// block GEMM per mid tuple whose A and B blocks both exist.  Here every set
+// is a sorted array of 64-bit keys built on the device:
+//   (qcode << 32 | tuple)   with qcode order = QN_t_2 order (util/QN.h:100)
+//   tuple = mixed-radix number of the positions, first position most
+//           significant, so integer order = std::set<uint_vec> order.
+// The planning kernels produce the result block list, its packed offsets, the
+// task list in the reference loop order and the grouped-GEMM tile list.
+#include <algorithm>
+#include <cmath>
+
+#include "ttg_device.h"
+#include "ttg_internal.h"
+#include "ttg_plan.h"
+
+namespace ttg {
+
+__device__ inline uint32_t qcode(int q0, int q1) {
+  return ((uint32_t)(q0 + 32768) << 16) | (uint32_t)(q1 + 32768);
+}
+
+struct KeyArrays {
+  uint64_t *kUL, *kSA, *kUR, *kUM, *kSB;     // raw keys
+  int *vSA, *vSB;                            // block ids
+  uint64_t *sUL, *sSA, *sUR, *sUM, *sSB;     // sorted
+  int *svSA, *svSB;
+};
+
+// ---- K1: per-block keys ------------------------------------------------
+__global__ void k_plan_keys(MetaView A, MetaView B, PairSpec ps, KeyArrays k) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < A.nblocks) {
+    const int* qi = A.qi + (size_t)i * A.rank;
+    int q0 = 0, q1 = 0;
+    uint64_t left = 0, mid = 0;
+    for (int j = 0; j < ps.nfa; j++) {
+      int p = ps.fa[j], s = qi[p];
+      const int* e = A.idx + 3 * (A.idx_off[p] + s);
+      int sg = A.arrow[p] == 0 ? 1 : -1;          // Inward +, Outward - (:617-621)
+      q0 += sg * e[0];
+      q1 += sg * e[1];
+      left = left * A.nsec[p] + s;
+    }
+    for (int t = 0; t < ps.np; t++) {
+      int p = ps.pa[t];
+      mid = mid * A.nsec[p] + qi[p];
+    }
+    k.kUL[i] = ((uint64_t)qcode(q0, q1) << 32) | left;
+    k.kSA[i] = (left << 32) | mid;
+    k.vSA[i] = i;
+  }
+  if (i < B.nblocks) {
+    const int* qi = B.qi + (size_t)i * B.rank;
+    int q0 = 0, q1 = 0;
+    uint64_t right = 0, mid = 0;
+    for (int j = 0; j < ps.nfb; j++) {
+      int p = ps.fb[j], s = qi[p];
+      const int* e = B.idx + 3 * (B.idx_off[p] + s);
+      int sg = B.arrow[p] == 0 ? -1 : 1;          // Inward -, Outward + (:629-633)
+      q0 += sg * e[0];
+      q1 += sg * e[1];
+      right = right * B.nsec[p] + s;
+    }
+    for (int t = 0; t < ps.np; t++) {
+      int p = ps.pb[t];
+      mid = mid * B.nsec[p] + qi[p];
+    }
+    uint64_t qc = (uint64_t)qcode(q0, q1) << 32;
+    k.kUR[i] = qc | right;
+    k.kUM[i] = qc | mid;
+    k.kSB[i] = (mid << 32) | right;
+    k.vSB[i] = i;
+  }
+}
+
+// ---- K2: stable rank sort (five arrays, grid.y selects) -----------------
+struct SortJob {
+  const uint64_t* in;
+  const int* vin;
+  uint64_t* out;
+  int* vout;
+  int n;
+};
+struct SortJobs { SortJob j[5]; };
+
+__global__ void __launch_bounds__(256) k_rank_sort(SortJobs jobs) {
+  const SortJob& J = jobs.j[blockIdx.y];
+  __shared__ uint64_t tile[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if ((int)(blockIdx.x * 256) >= J.n) return;
+  const uint64_t key = i < J.n ? J.in[i] : 0;
+  int rank = 0;
+  for (int base = 0; base < J.n; base += 256) {
+    __syncthreads();
+    if (base + (int)threadIdx.x < J.n) tile[threadIdx.x] = J.in[base + threadIdx.x];
+    __syncthreads();
+    const int lim = min(256, J.n - base);
+    for (int t = 0; t < lim; t++) {
+      uint64_t o = tile[t];
+      rank += (o < key) || (o == key && base + t < i);
+    }
+  }
+  if (i < J.n) {
+    J.out[rank] = key;
+    if (J.vout) J.vout[rank] = J.vin ? J.vin[i] : i;
+  }
+}
+
+// ---- K3: unique sets, per-q ranges, result-block bases -------------------
+struct QTab {
+  int *qL0, *qL1, *qR0, *qR1, *qM0, *qM1;
+  int64_t* qbase;   // nQ + 1
+};
+
+__device__ int compact_unique(const uint64_t* s, int n, uint64_t* u, int64_t* lds) {
+  int64_t carry = 0;
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + threadIdx.x;
+    int f = 0;
+    if (i < n) f = (i == 0) || (s[i] != s[i - 1]);
+    int64_t tot;
+    int64_t pos = block_excl_scan<1024>((int64_t)f, lds, &tot);
+    if (f) u[carry + pos] = s[i];
+    carry += tot;
+  }
+  __syncthreads();
+  return (int)carry;
+}
+
+__global__ void __launch_bounds__(1024) k_plan_unique(KeyArrays k, int nA, int nB, uint64_t* uL,
+                                                      uint64_t* uR, uint64_t* uM, QTab q,
+                                                      PlanHdr* hdr) {
+  __shared__ int64_t lds[32];
+  const int nuL = compact_unique(k.sUL, nA, uL, lds);
+  const int nuR = compact_unique(k.sUR, nB, uR, lds);
+  const int nuM = compact_unique(k.sUM, nB, uM, lds);
+  // q groups of uL (ascending q; A's mid-QN set, :622-623)
+  int64_t carry = 0;
+  for (int base = 0; base < nuL; base += 1024) {
+    int i = base + threadIdx.x;
+    int f = 0;
+    if (i < nuL) f = (i == 0) || ((uL[i] >> 32) != (uL[i - 1] >> 32));
+    int64_t tot;
+    int64_t pos = block_excl_scan<1024>((int64_t)f, lds, &tot);
+    if (f) q.qL0[carry + pos] = i;
+    carry += tot;
+  }
+  __syncthreads();
+  const int nQ = (int)carry;
+  for (int t = threadIdx.x; t < nQ; t += 1024) {
+    int l0 = q.qL0[t];
+    int l1 = (t + 1 < nQ) ? q.qL0[t + 1] : nuL;
+    q.qL1[t] = l1;
+    uint64_t qc = uL[l0] >> 32;
+    q.qR0[t] = lower_bound_u64(uR, nuR, qc << 32);
+    q.qR1[t] = lower_bound_u64(uR, nuR, (qc + 1) << 32);
+    q.qM0[t] = lower_bound_u64(uM, nuM, qc << 32);
+    q.qM1[t] = lower_bound_u64(uM, nuM, (qc + 1) << 32);
+  }
+  __syncthreads();
+  carry = 0;
+  for (int base = 0; base < nQ; base += 1024) {
+    int t = base + threadIdx.x;
+    int64_t cnt = 0;
+    if (t < nQ) cnt = (int64_t)(q.qL1[t] - q.qL0[t]) * (q.qR1[t] - q.qR0[t]);
+    int64_t tot;
+    int64_t pos = block_excl_scan<1024>(cnt, lds, &tot);
+    if (t < nQ) q.qbase[t] = carry + pos;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    q.qbase[nQ] = carry;
+    hdr->nuL = nuL;
+    hdr->nuR = nuR;
+    hdr->nuM = nuM;
+    hdr->nQ = nQ;
+    hdr->nblocksC = carry;
+  }
+}
+
+// ---- K4: per result block: tuple, dims, task enumeration ----------------
+struct BlkArrays {
+  int64_t* size;      // nblocksC
+  int64_t* ntask;     // nblocksC
+  double* flops;      // nblocksC
+  const int64_t* task_off;   // nblocksC + 1 (phase 1)
+  int* c_qi;          // nblocksC * rankC
+  const int64_t* c_off;      // nblocksC + 1 (phase 1)
+};
+struct EmitArrays {
+  GemmTask* tasks;
+  CBlock* cblk;
+  int* tuples;
+};
+
+__device__ inline int find_sorted(const uint64_t* a, int n, uint64_t key) {
+  int p = lower_bound_u64(a, n, key);
+  return (p < n && a[p] == key) ? p : -1;
+}
+
+// fold (extent, stride) components: drop extent-1, merge contiguous runs;
+// returns the component count (components beyond 2 are reported, not kept)
+__device__ inline int fold2(int n, const int* ext, const int64_t* st, int& e0, int64_t& s0,
+                            int64_t& s1) {
+  int cnt = 0;
+  int64_t ce = 1, cs = 0;
+  e0 = 1; s0 = 0; s1 = 0;
+  int ncomp = 0;
+  int64_t ce0 = 0;
+  for (int j = 0; j < n; j++) {
+    if (ext[j] == 1) continue;
+    if (cnt > 0 && st[j] == ce * cs) {
+      ce *= ext[j];
+      continue;
+    }
+    if (cnt > 0) {  // close previous
+      if (ncomp == 0) { ce0 = ce; s0 = cs; }
+      else if (ncomp == 1) { s1 = cs; }
+      ncomp++;
+    }
+    cnt = 1;
+    ce = ext[j];
+    cs = st[j];
+  }
+  if (cnt > 0) {
+    if (ncomp == 0) { ce0 = ce; s0 = cs; }
+    else if (ncomp == 1) { s1 = cs; }
+    ncomp++;
+  }
+  e0 = ncomp >= 1 ? (int)ce0 : 1;
+  return ncomp;
+}
+
+// K fold with two operands sharing the contraction index order
+__device__ inline int foldk(int n, const int* ext, const int64_t* sa, const int64_t* sb, int& e0,
+                            int64_t& a0, int64_t& a1, int64_t& b0, int64_t& b1) {
+  int ncomp = 0, open = 0;
+  int64_t ce = 1, ca = 0, cb = 0, ce0 = 1;
+  a0 = a1 = b0 = b1 = 0;
+  for (int j = 0; j < n; j++) {
+    if (ext[j] == 1) continue;
+    if (open && sa[j] == ce * ca && sb[j] == ce * cb) {
+      ce *= ext[j];
+      continue;
+    }
+    if (open) {
+      if (ncomp == 0) { ce0 = ce; a0 = ca; b0 = cb; }
+      else if (ncomp == 1) { a1 = ca; b1 = cb; }
+      ncomp++;
+    }
+    open = 1;
+    ce = ext[j]; ca = sa[j]; cb = sb[j];
+  }
+  if (open) {
+    if (ncomp == 0) { ce0 = ce; a0 = ca; b0 = cb; }
+    else if (ncomp == 1) { a1 = ca; b1 = cb; }
+    ncomp++;
+  }
+  e0 = ncomp >= 1 ? (int)ce0 : 1;
+  return ncomp;
+}
+
+__global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, PairSpec ps,
+                                                     KeyArrays k, const uint64_t* uL,
+                                                     const uint64_t* uR, const uint64_t* uM,
+                                                     QTab q, int nQ, int nblocksC, int phase,
+                                                     BlkArrays ba, EmitArrays ea, PlanHdr* hdr,
+                                                     int nA, int nB) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocksC) return;
+  // q group containing b: last t with qbase[t] <= b
+  int lo = 0, hi = nQ;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (q.qbase[mid] <= b) lo = mid + 1;
+    else hi = mid;
+  }
+  const int t = lo - 1;
+  const int64_t local = b - q.qbase[t];
+  const int nL = q.qL1[t] - q.qL0[t];
+  const int ri = (int)(local / nL), li = (int)(local % nL);
+  const uint64_t left = uL[q.qL0[t] + li] & 0xffffffffull;
+  const uint64_t right = uR[q.qR0[t] + ri] & 0xffffffffull;
+  // decode tuples (first position most significant)
+  int lq[MAXR], rq[MAXR];
+  {
+    uint64_t x = left;
+    for (int j = ps.nfa - 1; j >= 0; j--) { int p = ps.fa[j]; lq[j] = (int)(x % A.nsec[p]); x /= A.nsec[p]; }
+    x = right;
+    for (int j = ps.nfb - 1; j >= 0; j--) { int p = ps.fb[j]; rq[j] = (int)(x % B.nsec[p]); x /= B.nsec[p]; }
+  }
+  const int rankC = ps.nfa + ps.nfb;
+  int64_t M = 1, N = 1;
+  for (int j = 0; j < ps.nfa; j++) M *= A.idx[3 * (A.idx_off[ps.fa[j]] + lq[j]) + 2];
+  for (int j = 0; j < ps.nfb; j++) N *= B.idx[3 * (B.idx_off[ps.fb[j]] + rq[j]) + 2];
+  if (phase == 0) {
+    for (int j = 0; j < ps.nfa; j++) ba.c_qi[(size_t)b * rankC + j] = lq[j];
+    for (int j = 0; j < ps.nfb; j++) ba.c_qi[(size_t)b * rankC + ps.nfa + j] = rq[j];
+    ba.size[b] = M * N;
+  }
+  int64_t nt = 0;
+  double fl = 0.0;
+  int64_t tw = phase == 1 ? ba.task_off[b] : 0;
+  int64_t sumK = 0;
+  int votes[4] = {0, 0, 0, 0};
+  int bad = 0;
+  for (int m = q.qM0[t]; m < q.qM1[t]; m++) {
+    const uint64_t mid = uM[m] & 0xffffffffull;
+    const int ia = find_sorted(k.sSA, nA, (left << 32) | mid);
+    if (ia < 0) continue;
+    const int ib = find_sorted(k.sSB, nB, (mid << 32) | right);
+    if (ib < 0) continue;
+    const int ablk = k.svSA[ia], bblk = k.svSB[ib];
+    // strides of the stored (unpermuted) blocks
+    int dA[MAXR], dB[MAXR];
+    int64_t sA[MAXR], sB[MAXR];
+    int64_t acc = 1;
+    for (int j = 0; j < A.rank; j++) {
+      dA[j] = A.idx[3 * (A.idx_off[j] + A.qi[(size_t)ablk * A.rank + j]) + 2];
+      sA[j] = acc;
+      acc *= dA[j];
+    }
+    acc = 1;
+    for (int j = 0; j < B.rank; j++) {
+      dB[j] = B.idx[3 * (B.idx_off[j] + B.qi[(size_t)bblk * B.rank + j]) + 2];
+      sB[j] = acc;
+      acc *= dB[j];
+    }
+    int em[MAXR], ek[MAXR], en[MAXR];
+    int64_t smA[MAXR], skA[MAXR], skB[MAXR], snB[MAXR];
+    int64_t K = 1;
+    for (int j = 0; j < ps.nfa; j++) { em[j] = dA[ps.fa[j]]; smA[j] = sA[ps.fa[j]]; }
+    for (int p = 0; p < ps.np; p++) {
+      ek[p] = dB[ps.pb[p]];
+      skA[p] = sA[ps.pa[p]];
+      skB[p] = sB[ps.pb[p]];
+      K *= ek[p];
+    }
+    for (int j = 0; j < ps.nfb; j++) { en[j] = dB[ps.fb[j]]; snB[j] = sB[ps.fb[j]]; }
+    GemmTask g;
+    g.a_off = A.off[ablk];
+    g.b_off = B.off[bblk];
+    g.K = (int)K;
+    int cm = fold2(ps.nfa, em, smA, g.em0, g.sam0, g.sam1);
+    int cn = fold2(ps.nfb, en, snB, g.en0, g.sbn0, g.sbn1);
+    int ck = foldk(ps.np, ek, skA, skB, g.ek0, g.sak0, g.sak1, g.sbk0, g.sbk1);
+    if (ck > 2) {  // try the reversed contraction order
+      int ekr[MAXR];
+      int64_t skAr[MAXR], skBr[MAXR];
+      for (int p = 0; p < ps.np; p++) {
+        ekr[p] = ek[ps.np - 1 - p];
+        skAr[p] = skA[ps.np - 1 - p];
+        skBr[p] = skB[ps.np - 1 - p];
+      }
+      ck = foldk(ps.np, ekr, skAr, skBr, g.ek0, g.sak0, g.sak1, g.sbk0, g.sbk1);
+    }
+    if (cm > 2 || cn > 2 || ck > 2) bad = 1;
+    if (phase == 0) {
+      votes[0] += (g.sam0 == 1) || M == 1;
+      votes[1] += (g.sak0 == 1) || K == 1;
+      votes[2] += (g.sbk0 == 1) || K == 1;
+      votes[3] += (g.sbn0 == 1) || N == 1;
+    } else {
+      ea.tasks[tw + nt] = g;
+      if (ea.tuples) {
+        int* tu = ea.tuples + 6 * (tw + nt);
+        tu[0] = b; tu[1] = ablk; tu[2] = bblk; tu[3] = (int)M; tu[4] = (int)N; tu[5] = (int)K;
+      }
+    }
+    nt++;
+    sumK += K;
+    fl += 2.0 * (double)M * (double)N * (double)K;
+  }
+  if (phase == 0) {
+    ba.ntask[b] = nt;
+    ba.flops[b] = fl;
+    for (int c = 0; c < TILE_NCFG; c++) {
+      const int TMc = (c == 1 || c == 3) ? 32 : (c == 4 ? 128 : 64);
+      const int TNc = (c == 1 || c == 2) ? 32 : 64;
+      int64_t tm = (M + TMc - 1) / TMc, tn = (N + TNc - 1) / TNc;
+      atomicAdd((unsigned long long*)&hdr->tiles[c], (unsigned long long)(tm * tn));
+      atomicAdd(&hdr->padded[c], (double)(tm * TMc) * (double)(tn * TNc) * (double)sumK);
+    }
+    for (int v = 0; v < 4; v++)
+      if (votes[v]) atomicAdd((unsigned long long*)&hdr->votes[v], (unsigned long long)votes[v]);
+    if (bad) atomicAdd((unsigned long long*)&hdr->err, 1ull);
+  } else {
+    CBlock cb;
+    cb.c_off = ba.c_off[b];
+    cb.M = (int)M;
+    cb.N = (int)N;
+    cb.t0 = (int)tw;
+    cb.t1 = (int)(tw + nt);
+    ea.cblk[b] = cb;
+  }
+}
+
+// ---- K5: scans over result blocks ---------------------------------------
+__global__ void __launch_bounds__(1024) k_plan_scan(int n, const int64_t* size, const int64_t* ntask,
+                                                    const double* flops, int64_t* c_off,
+                                                    int64_t* task_off, PlanHdr* hdr) {
+  __shared__ int64_t lds[32];
+  __shared__ double red[16];
+  int64_t cs = 0, ct = 0;
+  double fsum = 0.0;
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + threadIdx.x;
+    int64_t s = i < n ? size[i] : 0, t = i < n ? ntask[i] : 0;
+    double f = i < n ? flops[i] : 0.0;
+    int64_t ts, tt;
+    int64_t ps = block_excl_scan<1024>(s, lds, &ts);
+    int64_t pt = block_excl_scan<1024>(t, lds, &tt);
+    if (i < n) {
+      c_off[i] = cs + ps;
+      task_off[i] = ct + pt;
+    }
+    fsum += block_sum<1024>(f, red);
+    cs += ts;
+    ct += tt;
+  }
+  if (threadIdx.x == 0) {
+    c_off[n] = cs;
+    task_off[n] = ct;
+    hdr->totalC = cs;
+    hdr->ntasks = ct;
+    hdr->flops = fsum;
+  }
+}
+
+// ---- K6: tile list (block order, row-major tiles inside a block) ----------
+__global__ void __launch_bounds__(1024) k_plan_tiles(int n, const CBlock* cblk, int TM, int TN,
+                                                     Tile* tiles) {
+  __shared__ int64_t lds[32];
+  int64_t carry = 0;
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + threadIdx.x;
+    int64_t cnt = 0, tm = 0, tn = 0;
+    if (i < n) {
+      tm = (cblk[i].M + TM - 1) / TM;
+      tn = (cblk[i].N + TN - 1) / TN;
+      cnt = tm * tn;
+    }
+    int64_t tot;
+    int64_t pos = block_excl_scan<1024>(cnt, lds, &tot);
+    if (i < n) {
+      int64_t w = carry + pos;
+      int cost = cblk[i].t1 - cblk[i].t0;
+      for (int64_t a = 0; a < tm; a++)
+        for (int64_t c = 0; c < tn; c++) {
+          Tile tl;
+          tl.c = i;
+          tl.m0 = (int)(a * TM);
+          tl.n0 = (int)(c * TN);
+          tl.cost = cost;
+          tiles[w++] = tl;
+        }
+    }
+    carry += tot;
+  }
+}
+
+}  // namespace ttg
+
+ttg_plan_s::~ttg_plan_s() {
+  ttg::dfree(d_tasks);
+  ttg::dfree(d_cblk);
+  ttg::dfree(d_tiles);
+  ttg::dfree(d_tuples);
+}
+
+namespace ttg {
+
+// Tile-shape choice: estimated time = padded MFMA work / peak + tile count x
+// per-tile fixed cost (epilogue, task descriptor loads), both in units of a
+// 64x64x16 k-step.  A 128x64 tile is only chosen for large blocks.
+static int choose_cfg(const PlanHdr& h) {
+  int best = 0;
+  double bestc = 1e300;
+  for (int c = 0; c < TILE_NCFG; c++) {
+    double area = (double)cfg_tm(c) * cfg_tn(c);
+    // smaller tiles run the MFMA pipe less efficiently (fewer MFMAs per
+    // LDS byte); scale their work by a penalty
+    double eff = (area >= 8192) ? 1.0 : (area >= 4096 ? 1.05 : (area >= 2048 ? 1.25 : 1.6));
+    double cost = h.padded[c] * eff / (64.0 * 64.0 * 16.0) + 6.0 * (double)h.tiles[c];
+    if (cost < bestc) {
+      bestc = cost;
+      best = c;
+    }
+  }
+  return best;
+}
+
+int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
+               int force_cfg, ttg_plan* out) {
+  TTG_TRY(ensure_init());
+  TTG_REQUIRE(np >= 0 && np <= A->rank && np <= B->rank, "plan: bad pair count");
+  PairSpec ps{};
+  ps.np = np;
+  std::vector<bool> usedA(A->rank, false), usedB(B->rank, false);
+  for (int p = 0; p < np; p++) {
+    TTG_REQUIRE(pa[p] >= 0 && pa[p] < A->rank && pb[p] >= 0 && pb[p] < B->rank,
+                "plan: pair position out of range");
+    TTG_REQUIRE(!usedA[pa[p]] && !usedB[pb[p]], "plan: index paired twice");
+    usedA[pa[p]] = usedB[pb[p]] = true;
+    ps.pa[p] = pa[p];
+    ps.pb[p] = pb[p];
+    // the pair must be the same index seen from both sides (tag match)
+    TTG_REQUIRE(A->nsec[pa[p]] == B->nsec[pb[p]], "plan: paired indices differ in sectors");
+    for (int s = 0; s < A->nsec[pa[p]]; s++)
+      TTG_REQUIRE(A->q0(pa[p], s) == B->q0(pb[p], s) && A->q1(pa[p], s) == B->q1(pb[p], s) &&
+                      A->dim(pa[p], s) == B->dim(pb[p], s),
+                  "plan: paired indices differ in sectors");
+  }
+  for (int j = 0; j < A->rank; j++) if (!usedA[j]) ps.fa[ps.nfa++] = j;
+  for (int j = 0; j < B->rank; j++) if (!usedB[j]) ps.fb[ps.nfb++] = j;
+  TTG_REQUIRE(ps.nfa + ps.nfb <= MAXR, "plan: result rank exceeds 8");
+  auto radix_ok = [](const TMeta& m, const int* pos, int n) {
+    double r = 1;
+    for (int j = 0; j < n; j++) r *= m.nsec[pos[j]];
+    return r < 4294967295.0;
+  };
+  TTG_REQUIRE(radix_ok(*A, ps.fa, ps.nfa) && radix_ok(*B, ps.fb, ps.nfb) && radix_ok(*A, ps.pa, np),
+              "plan: sector-tuple key exceeds 32 bits");
+
+  const int nA = A->nblocks, nB = B->nblocks;
+  hipStream_t s = stream();
+  // scratch
+  KeyArrays k{};
+  const int nmax = std::max(1, std::max(nA, nB));
+  k.kUL = dalloc<uint64_t>(nmax); k.kSA = dalloc<uint64_t>(nmax); k.kUR = dalloc<uint64_t>(nmax);
+  k.kUM = dalloc<uint64_t>(nmax); k.kSB = dalloc<uint64_t>(nmax);
+  k.sUL = dalloc<uint64_t>(nmax); k.sSA = dalloc<uint64_t>(nmax); k.sUR = dalloc<uint64_t>(nmax);
+  k.sUM = dalloc<uint64_t>(nmax); k.sSB = dalloc<uint64_t>(nmax);
+  k.vSA = dalloc<int>(nmax); k.vSB = dalloc<int>(nmax); k.svSA = dalloc<int>(nmax); k.svSB = dalloc<int>(nmax);
+  uint64_t* uL = dalloc<uint64_t>(nmax);
+  uint64_t* uR = dalloc<uint64_t>(nmax);
+  uint64_t* uM = dalloc<uint64_t>(nmax);
+  QTab q{};
+  q.qL0 = dalloc<int>(nmax); q.qL1 = dalloc<int>(nmax); q.qR0 = dalloc<int>(nmax);
+  q.qR1 = dalloc<int>(nmax); q.qM0 = dalloc<int>(nmax); q.qM1 = dalloc<int>(nmax);
+  q.qbase = dalloc<int64_t>(nmax + 1);
+  PlanHdr* d_hdr = dalloc<PlanHdr>(1);
+  PlanHdr* h_hdr = nullptr;
+  TTG_HIP(hipHostMalloc((void**)&h_hdr, sizeof(PlanHdr)));
+  TTG_HIP(hipMemsetAsync(d_hdr, 0, sizeof(PlanHdr), s));
+
+  auto release = [&]() {
+    dfree(k.kUL); dfree(k.kSA); dfree(k.kUR); dfree(k.kUM); dfree(k.kSB);
+    dfree(k.sUL); dfree(k.sSA); dfree(k.sUR); dfree(k.sUM); dfree(k.sSB);
+    dfree(k.vSA); dfree(k.vSB); dfree(k.svSA); dfree(k.svSB);
+    dfree(uL); dfree(uR); dfree(uM);
+    dfree(q.qL0); dfree(q.qL1); dfree(q.qR0); dfree(q.qR1); dfree(q.qM0); dfree(q.qM1);
+    dfree(q.qbase); dfree(d_hdr);
+    hipHostFree(h_hdr);
+  };
+
+  hipLaunchKernelGGL(k_plan_keys, dim3((nmax + 255) / 256), dim3(256), 0, s, A->view(), B->view(),
+                     ps, k);
+  SortJobs jobs;
+  jobs.j[0] = {k.kUL, nullptr, k.sUL, nullptr, nA};
+  jobs.j[1] = {k.kSA, k.vSA, k.sSA, k.svSA, nA};
+  jobs.j[2] = {k.kUR, nullptr, k.sUR, nullptr, nB};
+  jobs.j[3] = {k.kUM, nullptr, k.sUM, nullptr, nB};
+  jobs.j[4] = {k.kSB, k.vSB, k.sSB, k.svSB, nB};
+  hipLaunchKernelGGL(k_rank_sort, dim3((nmax + 255) / 256, 5), dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(k_plan_unique, dim3(1), dim3(1024), 0, s, k, nA, nB, uL, uR, uM, q, d_hdr);
+  TTG_HIP(hipMemcpyAsync(h_hdr, d_hdr, sizeof(PlanHdr), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  const int64_t nblocksC = h_hdr->nblocksC;
+  const int nQ = (int)h_hdr->nQ;
+  if (nblocksC >= (1 << 30)) {
+    release();
+    set_error("plan: too many result blocks");
+    return 1;
+  }
+
+  // result structure
+  std::vector<std::pair<const TMeta*, int>> cidx;
+  for (int j = 0; j < ps.nfa; j++) cidx.push_back({A.get(), ps.fa[j]});
+  for (int j = 0; j < ps.nfb; j++) cidx.push_back({B.get(), ps.fb[j]});
+  MetaPtr C = meta_from_indices(cidx);
+  if (!C) { release(); return 1; }
+  C->nblocks = (int)nblocksC;
+  C->d_qi = dalloc<int>((size_t)nblocksC * std::max(C->rank, 1));
+  C->d_off = dalloc<int64_t>((size_t)nblocksC + 1);
+  BlkArrays ba{};
+  ba.size = dalloc<int64_t>(nblocksC + 1);
+  ba.ntask = dalloc<int64_t>(nblocksC + 1);
+  ba.flops = dalloc<double>(nblocksC + 1);
+  int64_t* task_off = dalloc<int64_t>(nblocksC + 1);
+  ba.c_qi = C->d_qi;
+  EmitArrays ea{};
+  const unsigned gb = (unsigned)std::max<int64_t>(1, (nblocksC + 255) / 256);
+  hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
+                     uM, q, nQ, (int)nblocksC, 0, ba, ea, d_hdr, nA, nB);
+  hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, (int)nblocksC, ba.size, ba.ntask,
+                     ba.flops, C->d_off, task_off, d_hdr);
+  TTG_HIP(hipMemcpyAsync(h_hdr, d_hdr, sizeof(PlanHdr), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  if (h_hdr->err) {
+    dfree(ba.size); dfree(ba.ntask); dfree(ba.flops); dfree(task_off);
+    release();
+    set_error("plan: operand layout needs more than two strides per GEMM dimension");
+    return 1;
+  }
+  C->total = (size_t)h_hdr->totalC;
+
+  ttg_plan p = new ttg_plan_s();
+  p->A = A;
+  p->B = B;
+  p->C = C;
+  p->ps = ps;
+  p->prec = prec;
+  p->nblocksC = (int)nblocksC;
+  p->ntasks = h_hdr->ntasks;
+  p->flops = h_hdr->flops;
+  const int64_t nt = std::max<int64_t>(1, p->ntasks);
+  p->a_mcontig = h_hdr->votes[0] >= h_hdr->votes[1];
+  p->b_kcontig = h_hdr->votes[2] >= h_hdr->votes[3];
+  p->cfg = (force_cfg >= 0 && force_cfg < TILE_NCFG) ? force_cfg : choose_cfg(*h_hdr);
+  p->ntiles = (int)h_hdr->tiles[p->cfg];
+  p->d_tasks = dalloc<GemmTask>(nt);
+  p->d_cblk = dalloc<CBlock>(std::max<int64_t>(1, nblocksC));
+  p->d_tiles = dalloc<Tile>(std::max(1, p->ntiles));
+  p->d_tuples = dalloc<int>(6 * nt);
+  ba.task_off = task_off;
+  ba.c_off = C->d_off;
+  ea.tasks = p->d_tasks;
+  ea.cblk = p->d_cblk;
+  ea.tuples = p->d_tuples;
+  hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
+                     uM, q, nQ, (int)nblocksC, 1, ba, ea, d_hdr, nA, nB);
+  hipLaunchKernelGGL(k_plan_tiles, dim3(1), dim3(1024), 0, s, (int)nblocksC, p->d_cblk,
+                     cfg_tm(p->cfg), cfg_tn(p->cfg), p->d_tiles);
+  dfree(ba.size); dfree(ba.ntask); dfree(ba.flops); dfree(task_off);
+  release();
+  *out = p;
+  return 0;
+}
+
+}  // namespace ttg
+
+using namespace ttg;
+
+extern "C" {
+
+int ttg_contract_plan(ttg_tensor A, ttg_tensor B, int npairs, const int* pa, const int* pb, int prec,
+                      ttg_plan* out) {
+  TTG_REQUIRE(A && B && out, "contract_plan: null argument");
+  TTG_REQUIRE(prec == TTG_FP64 || prec == TTG_FP32 || prec == TTG_BF16X3, "unknown precision");
+  return plan_build(A->m, B->m, npairs, pa, pb, prec, -1, out);
+}
+
+int ttg_plan_info(ttg_plan p, int* nblocks, int64_t* ntasks, double* flops) {
+  TTG_REQUIRE(p, "null plan");
+  if (nblocks) *nblocks = p->nblocksC;
+  if (ntasks) *ntasks = p->ntasks;
+  if (flops) *flops = p->flops;
+  return 0;
+}
+
+int ttg_plan_tasks_download(ttg_plan p, int* tasks6) {
+  TTG_REQUIRE(p, "null plan");
+  if (p->ntasks)
+    TTG_HIP(hipMemcpyAsync(tasks6, p->d_tuples, sizeof(int) * 6 * (size_t)p->ntasks,
+                           hipMemcpyDeviceToHost, stream()));
+  TTG_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int ttg_plan_new_output(ttg_plan p, ttg_tensor* C) {
+  TTG_REQUIRE(p, "null plan");
+  return tensor_from_meta(p->C, false, C);
+}
+
+int ttg_contract_exec(ttg_plan p, ttg_tensor A, ttg_tensor B, ttg_tensor C) {
+  TTG_REQUIRE(p && A && B && C, "contract_exec: null argument");
+  TTG_REQUIRE((A->m == p->A || same_structure(A, A)) && A->m->nblocks == p->A->nblocks &&
+                  B->m->nblocks == p->B->nblocks && C->m->nblocks == p->C->nblocks,
+              "contract_exec: tensors do not match the plan");
+  if (p->prec == TTG_FP64) {
+    TTG_TRY(gemm_launch(*p, A->d_data, B->d_data, C->d_data, true, stream()));
+  } else {
+    const float *a32, *b32;
+    TTG_TRY(tensor_f32(A, &a32));
+    TTG_TRY(tensor_f32(B, &b32));
+    TTG_TRY(gemm_launch(*p, a32, b32, C->d_data, true, stream()));
+  }
+  mark_written(C);
+  return 0;
+}
+
+int ttg_plan_free(ttg_plan p) {
+  delete p;
+  return 0;
+}
+
+int ttg_contract(ttg_tensor A, ttg_tensor B, int npairs, const int* pa, const int* pb, int prec,
+                 ttg_tensor* C) {
+  ttg_plan p = nullptr;
+  TTG_TRY(ttg_contract_plan(A, B, npairs, pa, pb, prec, &p));
+  int r = ttg_plan_new_output(p, C);
+  if (r == 0) r = ttg_contract_exec(p, A, B, *C);
+  delete p;
+  return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// ttg_internal.h -- shared types of the gfx950 DMRG hot-path library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ttgpu.h"
+
+#define MAXR TTG_MAX_RANK
+
+namespace ttg {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+#define TTG_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t _e = (call);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      ::ttg::set_error(std::string(#call) + ": " + hipGetErrorString(_e));   \
+      return 1;                                                              \
+    }                                                                        \
+  } while (0)
+#define TTG_REQUIRE(cond, msg)        \
+  do {                                \
+    if (!(cond)) {                    \
+      ::ttg::set_error(msg);          \
+      return 1;                       \
+    }                                 \
+  } while (0)
+#define TTG_TRY(call)           \
+  do {                          \
+    int _r = (call);            \
+    if (_r != 0) return _r;     \
+  } while (0)
+
+// ---------------------------------------------------------------- runtime
+struct Runtime {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  int num_cu = 256;
+  bool inited = false;
+};
+Runtime& rt();
+int ensure_init();
+inline hipStream_t stream() { return rt().stream; }
+
+// caching device pool (size classes; buffers go back to the pool, reuse is
+// ordered by the library's single stream)
+void* dmalloc(size_t bytes);
+void dfree(void* p);
+template <class T>
+T* dalloc(size_t n) { return (T*)dmalloc(n * sizeof(T) + (n == 0 ? 8 : 0)); }
+
+// ---------------------------------------------------------------- tensors
+// Kernel-side view of a tensor's block structure.  Index sectors are stored
+// as (q0, q1, dim) int triples, all indices concatenated; idx_off[j] is the
+// first sector of index j.
+struct MetaView {
+  int rank;
+  int nblocks;
+  int arrow[MAXR];
+  int nsec[MAXR];
+  int idx_off[MAXR];
+  const int* idx;         // 3 ints per sector
+  const int* qi;          // nblocks * rank
+  const int64_t* off;     // nblocks + 1
+};
+
+// Immutable block structure shared by all tensors with the same block list.
+struct TMeta {
+  int rank = 0;
+  int nblocks = 0;
+  size_t total = 0;
+  int arrow[MAXR] = {0};
+  int nsec[MAXR] = {0};
+  int idx_off[MAXR + 1] = {0};
+  std::vector<int> h_idx;          // host copy of the (q0,q1,dim) triples
+  int* d_idx = nullptr;
+  int* d_qi = nullptr;
+  int64_t* d_off = nullptr;
+  // lazily built: sector-combination -> block id table (for embedding)
+  int* d_lut = nullptr;
+  int64_t lut_size = 0;
+  ~TMeta();
+  MetaView view() const;
+  int q0(int j, int s) const { return h_idx[3 * (idx_off[j] + s)]; }
+  int q1(int j, int s) const { return h_idx[3 * (idx_off[j] + s) + 1]; }
+  int dim(int j, int s) const { return h_idx[3 * (idx_off[j] + s) + 2]; }
+  bool same_indices(const TMeta& o) const;
+};
+using MetaPtr = std::shared_ptr<TMeta>;
+
+// index tables only (no blocks yet)
+int meta_shell(int rank, const int* arrows, const int* nsec, const int* qn, const int* qdim,
+               MetaPtr* out);
+// index tables copied from selected indices of other metas
+MetaPtr meta_from_indices(const std::vector<std::pair<const TMeta*, int>>& src);
+int meta_alloc_blocks(TMeta& m, int nblocks, size_t total);
+
+}  // namespace ttg
+
+struct ttg_tensor_s {
+  ttg::MetaPtr m;
+  double* d_data = nullptr;
+  float* d_f32 = nullptr;          // FP32 shadow of d_data, built on demand
+  uint64_t version = 1;            // bumped on every write of d_data
+  uint64_t f32_version = 0;
+  ~ttg_tensor_s();
+};
+
+namespace ttg {
+int tensor_from_meta(const MetaPtr& m, bool zero, ttg_tensor* out);
+inline void mark_written(ttg_tensor t) { t->version++; }
+int tensor_f32(ttg_tensor t, const float** out);
+bool same_structure(ttg_tensor a, ttg_tensor b);
+int block_scan_sum(const int64_t* in, int64_t* out_excl, int n);
+
+// device reductions shared by the vector ops
+int dev_dot(const double* x, const double* y, size_t n, double* d_out);
+}  // namespace ttg

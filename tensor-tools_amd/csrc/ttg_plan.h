@@ -1,0 +1,80 @@
+// ttg_plan.h -- contraction plan: device-built block list + grouped-GEMM work.
+#pragma once
+#include "ttg_internal.h"
+
+namespace ttg {
+
+// One matched block pair C_c += A_a * B_b (qtensor.cpp:719-721).  Operands
+// keep their stored column-major layout; each of the M, N and K ranges is
+// an (at most) two-component mixed-radix index with one stride per
+// component:  x -> (x % e0) * s0 + (x / e0) * s1.
+struct GemmTask {
+  int64_t a_off, b_off;
+  int64_t sam0, sam1, sak0, sak1;
+  int64_t sbk0, sbk1, sbn0, sbn1;
+  int em0, ek0, en0, K;
+};
+
+// Output block c: its K-contributions are tasks [t0, t1), written once.
+struct CBlock {
+  int64_t c_off;
+  int M, N;
+  int t0, t1;
+};
+
+struct Tile {
+  int c, m0, n0, cost;
+};
+
+struct PairSpec {
+  int np, nfa, nfb;
+  int pa[MAXR], pb[MAXR], fa[MAXR], fb[MAXR];
+};
+
+// tile shapes the planner chooses between (MFMA 16x16 building block)
+enum TileCfg { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_32x64 = 3, TILE_128x64 = 4,
+               TILE_NCFG = 5 };
+inline int cfg_tm(int c) { const int t[] = {64, 32, 64, 32, 128}; return t[c]; }
+inline int cfg_tn(int c) { const int t[] = {64, 32, 32, 64, 64}; return t[c]; }
+
+// device header filled by the planning kernels
+struct PlanHdr {
+  int64_t nuL, nuR, nuM, nQ, nblocksC;
+  int64_t totalC, ntasks;
+  double flops;
+  int64_t tiles[TILE_NCFG];
+  double padded[TILE_NCFG];      // sum of padded tile MACs (tile area x K)
+  int64_t votes[4];              // tasks with unit stride: A m, A k, B k, B n
+  int64_t err;
+};
+
+}  // namespace ttg
+
+struct ttg_plan_s {
+  ttg::MetaPtr A, B, C;          // structures the plan was built on / produces
+  ttg::PairSpec ps;
+  int prec = TTG_FP64;
+  int nblocksC = 0;
+  int64_t ntasks = 0;
+  double flops = 0;
+  int cfg = ttg::TILE_64x64;
+  int ntiles = 0;
+  bool a_mcontig = true, b_kcontig = true;
+  ttg::GemmTask* d_tasks = nullptr;
+  ttg::CBlock* d_cblk = nullptr;
+  ttg::Tile* d_tiles = nullptr;
+  // 6-tuples (c, a, b, M, N, K) for the bookkeeping checks
+  int* d_tuples = nullptr;
+  ~ttg_plan_s();
+};
+
+namespace ttg {
+int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
+               int force_cfg, ttg_plan* out);
+// launch the grouped GEMM of a plan.  Operand element types follow prec:
+// FP64 reads double A/B; FP32 and BF16X3 read float A/B.  out_f64 selects
+// a double or float C.
+int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
+                hipStream_t s);
+int gemm_grid(const ttg_plan_s& p);
+}  // namespace ttg

@@ -1,0 +1,260 @@
+// tt_gpu.hpp -- the reference's qtensor / big_qtensor / tensor_davidsonIT /
+// svd_bond / DMRG API (ClarkResearchGroup/tensor-tools) with device-resident
+// blocks.  Index descriptors, tags and pairing stay on the host (string
+// logic, as in the reference); block lists, offsets, block matching and all
+// arithmetic run on the GPU through the C-ABI in include/ttgpu.h.
+//
+// Differences from the reference that a caller can observe:
+//  * blocks live on one GPU; copies are device copies;
+//  * only qtensor<double> (the reference's complex svd is assert(1==2));
+//  * failures abort with the C-ABI's message (the reference asserts);
+//  * Davidson vectors carry the full legal block list of x's indices, so
+//    product outputs and Krylov vectors share one packed layout (the
+//    reference reaches the same block set through qtensor::add appends).
+#pragma once
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ttgpu.h"
+
+namespace ttgpu {
+
+[[noreturn]] void fail(const char* what);
+inline void check(int rc, const char* what) {
+  if (rc != 0) fail(what);
+}
+
+// util/QN.h:79-134
+struct QN_t {
+  int q0 = 0, q1 = 0;
+  QN_t() = default;
+  QN_t(int a, int b) : q0(a), q1(b) {}
+  QN_t operator+(const QN_t& o) const { return {q0 + o.q0, q1 + o.q1}; }
+  QN_t operator-(const QN_t& o) const { return {q0 - o.q0, q1 - o.q1}; }
+  QN_t operator-() const { return {-q0, -q1}; }
+  QN_t& operator+=(const QN_t& o) { q0 += o.q0; q1 += o.q1; return *this; }
+  QN_t& operator-=(const QN_t& o) { q0 -= o.q0; q1 -= o.q1; return *this; }
+  bool operator==(const QN_t& o) const { return q0 == o.q0 && q1 == o.q1; }
+  bool operator!=(const QN_t& o) const { return !(*this == o); }
+  bool operator<(const QN_t& o) const { return q0 < o.q0 || (q0 == o.q0 && q1 < o.q1); }
+};
+std::string to_string(const QN_t& q);   // "q0|q1|"
+
+enum arrow_type { Inward = 0, Outward = 1 };
+enum index_type { Link = 0, Site = 1, All = 2 };
+constexpr int MoveFromLeft = 0;
+constexpr int MoveFromRight = 1;
+
+// qtensor/qtensor_index.h:23-115
+class qtensor_index {
+ public:
+  qtensor_index() : qtensor_index(Inward, "qi", Link, 0) {}
+  explicit qtensor_index(arrow_type a, std::string name = "qi", index_type t = Link,
+                         unsigned level = 0);
+  void addQN(QN_t q, unsigned dim);
+  void rename(const std::string& n) { _name = n; setTag(); }
+  void retype(index_type t) { _type = t; setTag(); }
+  void dag() { _arrow = _arrow == Inward ? Outward : Inward; }
+  void prime(int inc = 1) { _level = (unsigned)((int)_level + inc); }
+  void primeLink(int inc = 1) { if (_type == Link) prime(inc); }
+  void primeSite(int inc = 1) { if (_type == Site) prime(inc); }
+  arrow_type arrow() const { return _arrow; }
+  unsigned size() const { return (unsigned)_qn.size(); }
+  unsigned level() const { return _level; }
+  const std::string& name() const { return _name; }
+  index_type type() const { return _type; }
+  QN_t qn(unsigned i) const { return _qn.at(i).first; }
+  unsigned qdim(unsigned i) const { return _qn.at(i).second; }
+  std::string tag() const;          // qtensor_index.h:66, arrow char-pointer quirk
+  std::string tagNoArrow() const;   // qtensor_index.h:67
+  bool operator==(const qtensor_index& o) const {
+    return _tag == o._tag && _arrow == o._arrow && _level == o._level;
+  }
+  bool operator!=(const qtensor_index& o) const { return !(*this == o); }
+  bool same_sectors(const qtensor_index& o) const { return _qn == o._qn; }
+  const std::vector<std::pair<QN_t, unsigned>>& sectors() const { return _qn; }
+
+ private:
+  void setTag();
+  arrow_type _arrow;
+  std::string _name;
+  index_type _type;
+  unsigned _level;
+  std::vector<std::pair<QN_t, unsigned>> _qn;
+  std::string _tag;
+};
+
+// the contraction-time index logic of qtensor::operator* (qtensor.cpp:536-607)
+struct IndexPairing {
+  std::vector<int> pa, pb;                 // contracted positions, reference order
+  std::vector<qtensor_index> result;       // A free (A order) ++ B free (B order)
+};
+IndexPairing pair_indices(const std::vector<qtensor_index>& A,
+                          const std::vector<qtensor_index>& B);
+
+// ---------------------------------------------------------------- qtensor
+using DevHandle = std::shared_ptr<ttg_tensor_s>;
+DevHandle adopt(ttg_tensor t);
+
+template <typename T>
+class qtensor;
+
+template <>
+class qtensor<double> {
+ public:
+  qtensor() = default;
+  explicit qtensor(const std::vector<qtensor_index>& idx) : rank((unsigned)idx.size()), idx_set(idx) {}
+  qtensor(std::initializer_list<qtensor_index> idx) : rank((unsigned)idx.size()), idx_set(idx) {}
+  qtensor(const qtensor& o);                 // device copy (qtensor.cpp:178-189)
+  qtensor(qtensor&& o) noexcept = default;
+  qtensor& operator=(const qtensor& o);
+  qtensor& operator=(qtensor&& o) noexcept = default;
+
+  void initBlock();                          // qtensor.cpp:302-345 (device)
+  void setZero();
+  void setOne();
+  void setRandom(uint64_t seed = 0x5eed);
+  qtensor operator*(const qtensor& B) const; // qtensor.cpp:523-734 (device)
+  qtensor& operator*=(double c);
+  qtensor& operator/=(double c) { return *this *= (1.0 / c); }
+  double contract(const qtensor& A) const;   // qtensor.cpp:1095-1119
+  void add(const qtensor& A, double c = 1.0);  // qtensor.cpp:1127-1156
+  double inner_product(const qtensor& A) const;  // qtensor.cpp:1161-1179
+  double norm() const;                       // qtensor.cpp:1606-1616
+  double normalize();                        // qtensor.cpp:1638-1647
+  void prime(int inc = 1) { for (auto& i : idx_set) i.prime(inc); }
+  void primeLink(int inc = 1) { for (auto& i : idx_set) i.primeLink(inc); }
+  void dag() { for (auto& i : idx_set) i.dag(); }
+  void conj() {}
+  size_t total_size() const;
+  int nblocks() const;
+
+  unsigned rank = 0;
+  std::vector<qtensor_index> idx_set;
+  bool _initted = false;
+  DevHandle dev;     // device blocks (null until initted)
+
+  // build the device tensor from host block tuples and packed data
+  void from_host(const std::vector<std::vector<unsigned>>& block_qi, const std::vector<double>& data);
+  std::vector<double> to_host() const;
+  std::vector<std::vector<unsigned>> block_list() const;
+};
+
+// precision of the effective-Hamiltonian contractions (TTG_FP64 default)
+void set_heff_precision(int prec);
+int heff_precision();
+
+// qtensor/big_qtensor.{h,cpp}
+template <typename T>
+class big_qtensor;
+
+template <>
+class big_qtensor<double> {
+ public:
+  big_qtensor() = default;
+  ~big_qtensor();
+  big_qtensor(const big_qtensor&) = delete;
+  big_qtensor& operator=(const big_qtensor&) = delete;
+  void setLeft(qtensor<double>* l) { L = l; drop(); }
+  void setRight(qtensor<double>* r) { R = r; drop(); }
+  void addMid(qtensor<double>* m);             // big_qtensor.h:56
+  qtensor<double> product(qtensor<double>& v); // big_qtensor.cpp:23-40
+  void product_into(qtensor<double>& v, qtensor<double>& y);
+  double expec(qtensor<double>& v);            // big_qtensor.cpp:45-64
+  size_t size() const;                         // big_qtensor.cpp:96-118
+  double flops_per_product() const;
+  // v re-laid onto the operator's block list (all legal blocks of v's indices)
+  qtensor<double> widen(const qtensor<double>& v);
+  ttg_heff op() const { return heff; }
+
+ private:
+  void drop();
+  void ensure_plan(const qtensor<double>& v);
+  qtensor<double>* L = nullptr;
+  qtensor<double>* R = nullptr;
+  qtensor<double> mid;
+  mutable size_t size_ = 0;
+  ttg_heff heff = nullptr;
+  qtensor<double> full;                         // layout the operator was planned for
+  int plan_prec = -1;
+};
+bool same_blocks(const qtensor<double>& a, const qtensor<double>& b);
+
+// linalg/tensor_davidson.cpp:192-347
+struct DavidsonStats { int n_matvec = 0; int iterations = 0; double residual = 0; };
+double tensor_davidsonIT(big_qtensor<double>& A, qtensor<double>& x, int m, int max_restart,
+                         double tol, char mode, DavidsonStats* st = nullptr);
+
+// qtensor/qtensor_op.cpp:1097-1544
+double svd(qtensor<double>& A, std::vector<qtensor_index>& left, std::vector<qtensor_index>& right,
+           qtensor<double>& U, qtensor<double>& V, qtensor<double>& S, int direction,
+           double cutoff = 0, unsigned K = 0, unsigned* kept = nullptr);
+double svd_bond(qtensor<double>& combined, qtensor<double>& A_left, qtensor<double>& A_right,
+                qtensor_index& mid, qtensor<double>& S, int direction, double cutoff = 0,
+                long unsigned K = 0, unsigned* kept = nullptr);
+double calcEntropy(qtensor<double>& S, double cutoff = 1e-24);
+
+// ---------------------------------------------------------------- MPS/MPO
+template <typename T, unsigned N>
+class qTensorTrain;
+
+template <unsigned N>
+class qTensorTrain<double, N> {
+ public:
+  unsigned _id = 0;
+  unsigned length = 0;
+  unsigned phy_dim = 2;
+  std::vector<unsigned> bond_dims;
+  int center = -1;
+  QN_t totalQ;
+  std::vector<QN_t> phy_qn;
+  std::vector<qtensor<double>> A;
+  bool tensors_allocated = false;
+  void rc();
+  void lc();
+  double position(int site);
+  void normalize();
+};
+template <typename T>
+using qMPS = qTensorTrain<T, 1>;
+template <typename T>
+using qMPO = qTensorTrain<T, 2>;
+
+// product-state MPS of spin-1/2 sites (qtt.cpp:60-78, 205-311); state[i]:
+// 0 = Dn, 1 = Up
+qMPS<double> product_state_mps(unsigned id, const std::vector<unsigned>& state);
+
+// models/hams: spin-1/2 Heisenberg MPOs (QN = (2Sz, 0))
+struct SpinTerm { double coef; std::string op1; int i1; std::string op2; int i2; };
+qMPO<double> heisenberg_mpo_nn(unsigned N, unsigned id);                     // Heisenberg.cpp:333-402
+qMPO<double> autompo_mpo(unsigned N, unsigned id, const std::vector<SpinTerm>& terms);  // :416-635
+std::vector<std::pair<int, int>> square_lattice(int Nx, int Ny, bool yperiodic,
+                                                std::vector<int>* type, bool nnn);
+std::vector<SpinTerm> heisenberg_terms(const std::vector<std::pair<int, int>>& bonds,
+                                       const std::vector<double>& J);
+
+// algos/dmrg/dmrg.cpp
+void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+              std::vector<qtensor<double>>& TL);
+void updateEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+               std::vector<qtensor<double>>& TL, const unsigned& site, int& direction);
+struct SiteLog { double energy, trunc_err; unsigned kept; int n_matvec; double t_dav, t_svd, t_env; };
+void updateSite(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+                std::vector<qtensor<double>>& TL, const unsigned& site, double& energy,
+                int& direction, int max_bd, double cutoff, char mode, int search_space_size,
+                int max_restart, double tol = 1e-12, SiteLog* log = nullptr);
+struct DmrgLog {
+  double initial_energy = 0;
+  std::vector<double> halfsweep_energy, halfsweep_seconds, bond_energy, trunc_err;
+  double t_dav = 0, t_svd = 0, t_env = 0;
+  long n_matvec = 0;
+};
+double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
+            const std::vector<double>& cutoff, const std::vector<int>& max_restart,
+            double tol = 1e-12, DmrgLog* log = nullptr);
+
+}  // namespace ttgpu
