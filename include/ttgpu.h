@@ -59,6 +59,9 @@ int ttg_tensor_info(ttg_tensor t, int* rank, int* nblocks, size_t* total);
 /* block list and offsets (nblocks+1 entries) of a tensor, for the bit-exact
  * bookkeeping checks */
 int ttg_tensor_meta_download(ttg_tensor t, int* block_qi, int64_t* offsets);
+/* qtensor::dag (qtensor.cpp:1575-1581): flip every index arrow.  Blocks
+ * and data are unchanged (a zero-flux block stays zero-flux). */
+int ttg_tensor_dag(ttg_tensor t);
 /* *same = 1 when a and b have identical index sectors and block lists
  * (the precondition of the packed-array vector ops below) */
 int ttg_same_blocks(ttg_tensor a, ttg_tensor b, int* same);
@@ -116,6 +119,12 @@ int ttg_heff_info(ttg_heff h, double* flops, int64_t* ntasks3, double* stage_flo
 /* device time of the last n applies measured with HIP events around the
  * graph launch on the library stream (ms per apply) */
 int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms_per_apply);
+/* per-stage device time (ms, mean over n runs) of the three grouped-GEMM
+ * launches, HIP events on the library stream around each launch */
+int ttg_heff_stage_time(ttg_heff h, ttg_tensor x, int n, double* ms3);
+/* tile shape index (64x64, 32x32, 64x32, 32x64, 128x64) and tile count
+ * the planner chose for each stage */
+int ttg_heff_tiles(ttg_heff h, int* tile_cfg3, int* ntiles3);
 int ttg_heff_free(ttg_heff h);
 
 /* ---- sector SVD with global truncation (qtensor_op.cpp:1097-1453) -------

@@ -273,7 +273,7 @@ def dense_svd(A: np.ndarray):
 
 def diag(M: np.ndarray):
     n = M.shape[0]
-    a = np.ascontiguousarray(np.asfortranarray(M).reshape(-1, order="F"))
+    a = np.array(M, dtype=np.float64).reshape(-1, order="F").copy()
     ev = np.zeros(n)
     lib().orc_diag(n, a, ev)
     return ev, a.reshape((n, n), order="F")

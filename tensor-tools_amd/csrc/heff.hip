@@ -236,6 +236,49 @@ int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms) {
   return 0;
 }
 
+int ttg_heff_stage_time(ttg_heff h, ttg_tensor x, int n, double* ms3) {
+  TTG_REQUIRE(h && x && n > 0 && ms3, "heff_stage_time: bad argument");
+  hipStream_t s = stream();
+  const size_t bytes = h->xm->total * sizeof(double);
+  TTG_HIP(hipMemcpyAsync(h->xbuf, x->d_data, bytes, hipMemcpyDeviceToDevice, s));
+  const bool red = h->prec != TTG_FP64;
+  if (red) hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(h->xm->total)), dim3(256), 0, s, h->xbuf,
+                              h->x32, h->xm->total);
+  hipEvent_t ev[4];
+  for (auto& e : ev) TTG_HIP(hipEventCreate(&e));
+  float acc[3] = {0, 0, 0};
+  for (int it = 0; it < n; it++) {
+    TTG_HIP(hipEventRecord(ev[0], s));
+    if (red) TTG_TRY(gemm_launch(*h->p[0], h->L32, h->x32, h->T1, false, s));
+    else TTG_TRY(gemm_launch(*h->p[0], h->L->d_data, h->xbuf, h->T1, true, s));
+    TTG_HIP(hipEventRecord(ev[1], s));
+    if (red) TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W32, h->T2, false, s));
+    else TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W->d_data, h->T2, true, s));
+    TTG_HIP(hipEventRecord(ev[2], s));
+    if (red) TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R32, h->ybuf, true, s));
+    else TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R->d_data, h->ybuf, true, s));
+    TTG_HIP(hipEventRecord(ev[3], s));
+    TTG_HIP(hipEventSynchronize(ev[3]));
+    for (int k = 0; k < 3; k++) {
+      float t = 0;
+      TTG_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      acc[k] += t;
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  for (int k = 0; k < 3; k++) ms3[k] = acc[k] / n;
+  return 0;
+}
+
+int ttg_heff_tiles(ttg_heff h, int* tile_cfg3, int* ntiles3) {
+  TTG_REQUIRE(h, "null heff");
+  for (int k = 0; k < 3; k++) {
+    if (tile_cfg3) tile_cfg3[k] = h->p[k]->cfg;
+    if (ntiles3) ntiles3[k] = h->p[k]->ntiles;
+  }
+  return 0;
+}
+
 int ttg_heff_free(ttg_heff h) {
   if (h) hipStreamSynchronize(stream());
   delete h;

@@ -1,13 +1,781 @@
 // svd.hip -- sector SVD with global truncation (qtensor_op.cpp:1097-1453).
+//
+// Per mid-QN sector q (flux of the left indices, ascending QN order) the
+// reference assembles a dense matrix from the blocks (rows: left position
+// tuples in lexicographic order, columns: right tuples), takes its SVD,
+// absorbs S into V (MoveFromLeft) or U (MoveFromRight), sorts all singular
+// values globally (descending), keeps them while 1 - cumsum(s^2/sum s^2)
+// >= cutoff (stopping at exact zeros, capping at K) and slices U, S, V.
+//
+// Device form (all FP64):
+//   1. bookkeeping kernels: per-block sector keys, sorted unique row / column
+//      tuple lists, sector sizes and row / column offsets;
+//   2. gather of the blocks into packed sector matrices;
+//   3. Gram matrix per sector on the grouped MFMA GEMM: A*A^T for
+//      MoveFromLeft (the left factor U is what must be orthonormal),
+//      A^T*A for MoveFromRight;
+//   4. one workgroup per sector: parallel cyclic Jacobi eigensolver
+//      (round-robin pairs, packed upper triangle in LDS when it fits) with
+//      the eigenvectors accumulated in HBM;  s = sqrt(max(lambda, 0));
+//   5. truncation on the device: per-sector rank-K cap, sum of s^2 and the
+//      sequential cumsum rule in one thread, in the reference's order;
+//   6. the absorbed factor as a second grouped GEMM (U^T A or A V), then the
+//      slicing of U, V and S into blocks.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "ttg_device.h"
 #include "ttg_internal.h"
+#include "ttg_plan.h"
+
+namespace ttg {
+
+__device__ inline uint32_t qcode_s(int q0, int q1) {
+  return ((uint32_t)(q0 + 32768) << 16) | (uint32_t)(q1 + 32768);
+}
+
+// 1a. keys: (q << 32 | left tuple), (q << 32 | right tuple)
+__global__ void k_svd_keys(MetaView A, int nleft, uint64_t* kl, uint64_t* kr, int* ids) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= A.nblocks) return;
+  const int* qi = A.qi + (size_t)b * A.rank;
+  int q0 = 0, q1 = 0;
+  uint64_t l = 0, r = 0;
+  for (int j = 0; j < A.rank; j++) {
+    if (j < nleft) {
+      const int* e = A.idx + 3 * (A.idx_off[j] + qi[j]);
+      int sg = A.arrow[j] == 0 ? 1 : -1;   // qtensor_op.cpp:1129-1133
+      q0 += sg * e[0];
+      q1 += sg * e[1];
+      l = l * A.nsec[j] + qi[j];
+    } else {
+      r = r * A.nsec[j] + qi[j];
+    }
+  }
+  uint64_t qc = (uint64_t)qcode_s(q0, q1) << 32;
+  kl[b] = qc | l;
+  kr[b] = qc | r;
+  ids[b] = b;
+}
+
+// size (product of sector dims) of a tuple over indices [j0, j1)
+__device__ inline int64_t tuple_size(const MetaView& A, int j0, int j1, uint64_t key) {
+  int64_t s = 1;
+  for (int j = j1 - 1; j >= j0; j--) {
+    int p = (int)(key % A.nsec[j]);
+    key /= A.nsec[j];
+    s *= A.idx[3 * (A.idx_off[j] + p) + 2];
+  }
+  return s;
+}
+
+struct SvdTab {
+  uint64_t *uL, *uR;          // unique sorted keys
+  int64_t *lsz, *loff;        // per uL entry: size, row offset inside its sector
+  int64_t *rsz, *roff;        // per uR entry
+  int *qL0, *qL1, *qR0, *qR1; // per sector ranges
+  int64_t *rows, *cols;       // per sector
+  uint32_t* qc;               // per sector QN code
+  int64_t* hdr;               // [nuL, nuR, nQ]
+};
+
+// 1b. unique lists, sector table, offsets (one workgroup)
+__global__ void __launch_bounds__(1024) k_svd_unique(MetaView A, int nleft, const uint64_t* sl,
+                                                     const uint64_t* sr, int n, SvdTab t) {
+  __shared__ int64_t lds[32];
+  int64_t carry = 0;
+  for (int base = 0; base < n; base += 1024) {   // unique left
+    int i = base + threadIdx.x;
+    int f = i < n && (i == 0 || sl[i] != sl[i - 1]);
+    int64_t tot;
+    int64_t pos = block_excl_scan<1024>((int64_t)f, lds, &tot);
+    if (f) t.uL[carry + pos] = sl[i];
+    carry += tot;
+  }
+  __syncthreads();
+  const int nuL = (int)carry;
+  carry = 0;
+  for (int base = 0; base < n; base += 1024) {   // unique right
+    int i = base + threadIdx.x;
+    int f = i < n && (i == 0 || sr[i] != sr[i - 1]);
+    int64_t tot;
+    int64_t pos = block_excl_scan<1024>((int64_t)f, lds, &tot);
+    if (f) t.uR[carry + pos] = sr[i];
+    carry += tot;
+  }
+  __syncthreads();
+  const int nuR = (int)carry;
+  carry = 0;
+  for (int base = 0; base < nuL; base += 1024) {  // sectors from uL
+    int i = base + threadIdx.x;
+    int f = i < nuL && (i == 0 || (t.uL[i] >> 32) != (t.uL[i - 1] >> 32));
+    int64_t tot;
+    int64_t pos = block_excl_scan<1024>((int64_t)f, lds, &tot);
+    if (f) t.qL0[carry + pos] = i;
+    carry += tot;
+  }
+  __syncthreads();
+  const int nQ = (int)carry;
+  for (int q = threadIdx.x; q < nQ; q += 1024) {
+    int l0 = t.qL0[q], l1 = q + 1 < nQ ? t.qL0[q + 1] : nuL;
+    t.qL1[q] = l1;
+    uint64_t qc = t.uL[l0] >> 32;
+    t.qc[q] = (uint32_t)qc;
+    t.qR0[q] = lower_bound_u64(t.uR, nuR, qc << 32);
+    t.qR1[q] = lower_bound_u64(t.uR, nuR, (qc + 1) << 32);
+  }
+  __syncthreads();
+  // per-entry sizes and offsets inside the sector (sequential per sector)
+  for (int q = threadIdx.x; q < nQ; q += 1024) {
+    int64_t o = 0;
+    for (int i = t.qL0[q]; i < t.qL1[q]; i++) {
+      int64_t s = tuple_size(A, 0, nleft, t.uL[i] & 0xffffffffull);
+      t.lsz[i] = s;
+      t.loff[i] = o;
+      o += s;
+    }
+    t.rows[q] = o;
+    o = 0;
+    for (int i = t.qR0[q]; i < t.qR1[q]; i++) {
+      int64_t s = tuple_size(A, nleft, A.rank, t.uR[i] & 0xffffffffull);
+      t.rsz[i] = s;
+      t.roff[i] = o;
+      o += s;
+    }
+    t.cols[q] = o;
+  }
+  if (threadIdx.x == 0) {
+    t.hdr[0] = nuL;
+    t.hdr[1] = nuR;
+    t.hdr[2] = nQ;
+  }
+}
+
+// 2. gather blocks into sector matrices (column-major rows_q x cols_q)
+__global__ void k_svd_gather(MetaView A, int nleft, const double* __restrict__ a, SvdTab t,
+                             int nuL, int nuR, int nQ, const int64_t* moff,
+                             double* __restrict__ mat) {
+  for (int b = blockIdx.x; b < A.nblocks; b += gridDim.x) {
+    const int* qi = A.qi + (size_t)b * A.rank;
+    int q0 = 0, q1 = 0;
+    uint64_t l = 0, r = 0;
+    for (int j = 0; j < A.rank; j++) {
+      if (j < nleft) {
+        const int* e = A.idx + 3 * (A.idx_off[j] + qi[j]);
+        int sg = A.arrow[j] == 0 ? 1 : -1;
+        q0 += sg * e[0];
+        q1 += sg * e[1];
+        l = l * A.nsec[j] + qi[j];
+      } else {
+        r = r * A.nsec[j] + qi[j];
+      }
+    }
+    const uint64_t qc = (uint64_t)qcode_s(q0, q1) << 32;
+    const int il = lower_bound_u64(t.uL, nuL, qc | l);
+    const int ir = lower_bound_u64(t.uR, nuR, qc | r);
+    // sector of il
+    int lo = 0, hi = nQ;
+    while (lo < hi) {
+      int m = (lo + hi) >> 1;
+      if (t.qL0[m] <= il) lo = m + 1;
+      else hi = m;
+    }
+    const int q = lo - 1;
+    const int64_t rows = t.rows[q];
+    const int64_t ls = t.lsz[il], rs = t.rsz[ir];
+    double* dst = mat + moff[q] + t.loff[il] + rows * t.roff[ir];
+    const double* src = a + A.off[b];
+    const int64_t n = ls * rs;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+      const int64_t i = e % ls, j = e / ls;
+      dst[i + rows * j] = src[e];
+    }
+  }
+}
+
+// 4. parallel cyclic Jacobi, one workgroup per sector.  n even (padded with
+// a zero row/column when odd; the pad never rotates).  Packed upper triangle
+// (i <= j at j(j+1)/2 + i) in LDS when it fits, else in HBM scratch.
+constexpr int kJacThreads = 1024;
+constexpr int kLdsDoubles = 18000;   // 141 KB of packed triangle (+12 KB static)
+
+__device__ inline int pk(int i, int j) { return i <= j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j; }
+
+__global__ void __launch_bounds__(kJacThreads)
+k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
+         const double* __restrict__ Gin, double* __restrict__ gscratch,
+         const int64_t* __restrict__ poff, double* __restrict__ Vout,
+         double* __restrict__ evals, const int64_t* __restrict__ eoff, int max_sweeps) {
+  extern __shared__ double lds_g[];
+  __shared__ double cs_c[512], cs_s[512];
+  __shared__ int pp[512], qq[512];
+  __shared__ int any_rot;
+  const int q = blockIdx.x;
+  const int n = nq[q];                 // true size
+  if (n == 0) return;
+  const int np = n + (n & 1);          // padded even size
+  const int npair = np / 2;
+  const int64_t psize = (int64_t)np * (np + 1) / 2;
+  double* g = (psize <= kLdsDoubles) ? lds_g : gscratch + poff[q];
+  const double* G0 = Gin + goff[q];    // full n x n column-major
+  double* V = Vout + goff[q];          // n x n column-major eigenvectors
+  // load packed triangle (pad row/col zero), V = I
+  for (int64_t e = threadIdx.x; e < psize; e += kJacThreads) {
+    int j = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
+    while ((int64_t)j * (j + 1) / 2 > e) j--;
+    while ((int64_t)(j + 1) * (j + 2) / 2 <= e) j++;
+    int i = (int)(e - (int64_t)j * (j + 1) / 2);
+    g[e] = (i < n && j < n) ? 0.5 * (G0[i + (int64_t)n * j] + G0[j + (int64_t)n * i]) : 0.0;
+  }
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kJacThreads)
+    V[e] = (e % n == e / n) ? 1.0 : 0.0;
+  __syncthreads();
+  const int m1 = np - 1;
+  for (int sweep = 0; sweep < max_sweeps; sweep++) {
+    if (threadIdx.x == 0) any_rot = 0;
+    __syncthreads();
+    for (int round = 0; round < m1; round++) {
+      // circle method: player m1 fixed, pairs (round, m1) and (round+k, round-k)
+      for (int k = threadIdx.x; k < npair; k += kJacThreads) {
+        int a, b;
+        if (k == 0) { a = round; b = m1; }
+        else { a = (round + k) % m1; b = (round - k + m1) % m1; }
+        if (a > b) { int t = a; a = b; b = t; }
+        pp[k] = a;
+        qq[k] = b;
+        const double app = g[pk(a, a)], aqq = g[pk(b, b)], apq = g[pk(a, b)];
+        double c = 1.0, s = 0.0;
+        if (apq != 0.0) {
+          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) any_rot = 1;
+          const double theta = (aqq - app) / (2.0 * apq);
+          const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          c = 1.0 / sqrt(t * t + 1.0);
+          s = t * c;
+        }
+        cs_c[k] = c;
+        cs_s[k] = s;
+      }
+      __syncthreads();
+      // G <- J^T G J over 2x2 blocks (k1 <= k2) of the packed triangle
+      const int nblk = npair * (npair + 1) / 2;
+      for (int w = threadIdx.x; w < nblk; w += kJacThreads) {
+        int k2 = (int)((sqrt(8.0 * (double)w + 1.0) - 1.0) / 2.0);
+        while (k2 * (k2 + 1) / 2 > w) k2--;
+        while ((k2 + 1) * (k2 + 2) / 2 <= w) k2++;
+        const int k1 = w - k2 * (k2 + 1) / 2;
+        const int a = pp[k1], b = qq[k1], c2i = pp[k2], d = qq[k2];
+        const double c1 = cs_c[k1], s1 = cs_s[k1], c2 = cs_c[k2], s2 = cs_s[k2];
+        if (k1 == k2) {
+          const double gaa = g[pk(a, a)], gab = g[pk(a, b)], gbb = g[pk(b, b)];
+          // rows then columns of the symmetric 2x2 block
+          const double ra0 = c1 * gaa - s1 * gab, ra1 = c1 * gab - s1 * gbb;
+          const double rb0 = s1 * gaa + c1 * gab, rb1 = s1 * gab + c1 * gbb;
+          g[pk(a, a)] = c1 * ra0 - s1 * ra1;
+          g[pk(a, b)] = s1 * ra0 + c1 * ra1;
+          g[pk(b, b)] = s1 * rb0 + c1 * rb1;
+        } else {
+          const int iac = pk(a, c2i), iad = pk(a, d), ibc = pk(b, c2i), ibd = pk(b, d);
+          const double gac = g[iac], gad = g[iad], gbc = g[ibc], gbd = g[ibd];
+          const double ra_c = c1 * gac - s1 * gbc, ra_d = c1 * gad - s1 * gbd;
+          const double rb_c = s1 * gac + c1 * gbc, rb_d = s1 * gad + c1 * gbd;
+          g[iac] = c2 * ra_c - s2 * ra_d;
+          g[iad] = s2 * ra_c + c2 * ra_d;
+          g[ibc] = c2 * rb_c - s2 * rb_d;
+          g[ibd] = s2 * rb_c + c2 * rb_d;
+        }
+      }
+      // V <- V J (columns of real indices only)
+      for (int w = threadIdx.x; w < npair * n; w += kJacThreads) {
+        const int k = w / n, i = w % n;
+        const int a = pp[k], b = qq[k];
+        if (b >= n) continue;   // pad never rotates (apq == 0)
+        const double c = cs_c[k], s = cs_s[k];
+        const double va = V[i + (int64_t)n * a], vb = V[i + (int64_t)n * b];
+        V[i + (int64_t)n * a] = c * va - s * vb;
+        V[i + (int64_t)n * b] = s * va + c * vb;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    if (!any_rot) break;
+  }
+  for (int i = threadIdx.x; i < n; i += kJacThreads) evals[eoff[q] + i] = g[pk(i, i)];
+}
+
+// 5a. per sector: eigen order (descending), top-k singular values in the
+// reference's sector / descending order, selected eigenvectors packed
+__global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq,
+                                                    const int64_t* eoff, const double* evals,
+                                                    const int64_t* goff, const double* V,
+                                                    const int64_t* soff, double* svals,
+                                                    int* ssec, const int64_t* selo,
+                                                    double* Esel) {
+  const int q = blockIdx.x;
+  const int n = nq[q], k = kq[q];
+  const double* ev = evals + eoff[q];
+  // rank of each eigenvalue (descending, ties by index): rank < k selected
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = ev[i];
+    int r = 0;
+    for (int j = 0; j < n; j++) r += (ev[j] > v) || (ev[j] == v && j < i);
+    if (r < k) {
+      svals[soff[q] + r] = sqrt(fmax(v, 0.0));
+      ssec[soff[q] + r] = q;
+      const double* src = V + goff[q] + (int64_t)n * i;
+      double* dst = Esel + selo[q] + (int64_t)n * r;
+      for (int e = 0; e < n; e++) dst[e] = src[e];
+    }
+  }
+}
+
+// 5b. truncation (qtensor_op.cpp:1280-1301): nm and cumsum are summed
+// sequentially in the reference's order; the global order is a stable
+// descending sort (the reference's std::sort is unstable on ties).
+__global__ void __launch_bounds__(256) k_svd_truncate(int T, const double* svals, const int* ssec,
+                                                      int* order, double cutoff, int64_t K,
+                                                      int* newdim, double* out) {
+  for (int i = threadIdx.x; i < T; i += blockDim.x) {
+    const double v = svals[i];
+    int r = 0;
+    for (int j = 0; j < T; j++) r += (svals[j] > v) || (svals[j] == v && j < i);
+    order[r] = i;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double nm = 0.0;
+    for (int i = 0; i < T; i++) nm += svals[i] * svals[i];
+    double cumsum = 0.0;
+    int64_t acc = 0;
+    for (int i = 0; i < T; i++) {
+      const double s = svals[order[i]];
+      cumsum += s * s / nm;
+      if (s == 0. && cutoff != 0.) break;
+      newdim[ssec[order[i]]] += 1;
+      acc += 1;
+      if ((1 - cumsum < cutoff && cutoff != 0.) || (acc >= K && K != 0)) break;
+    }
+    out[0] = fmax(1 - cumsum, 0.0);
+    out[1] = (double)acc;
+  }
+}
+
+// 6a. block lists of U (left..., mid) and V (mid, right...) and S (mid)
+__global__ void __launch_bounds__(1024) k_svd_emit(MetaView A, int nleft, SvdTab t, int nQ,
+                                                   const int* newdim, const int* midnum,
+                                                   int* uqi, int64_t* uoff, int* vqi,
+                                                   int64_t* voff) {
+  __shared__ int64_t lds[32];
+  const int nuL = (int)t.hdr[0], nuR = (int)t.hdr[1];
+  const int rU = nleft + 1, rV = A.rank - nleft + 1;
+  // U: entries of uL whose sector keeps values, in (sector, left) order
+  int64_t cb = 0, co = 0;
+  for (int base = 0; base < nuL; base += 1024) {
+    int i = base + threadIdx.x;
+    int q = -1;
+    int64_t f = 0, sz = 0;
+    if (i < nuL) {
+      int lo = 0, hi = nQ;
+      while (lo < hi) { int m = (lo + hi) >> 1; if (t.qL0[m] <= i) lo = m + 1; else hi = m; }
+      q = lo - 1;
+      f = newdim[q] > 0;
+      sz = f ? t.lsz[i] * newdim[q] : 0;
+    }
+    int64_t tb, ts;
+    int64_t pb = block_excl_scan<1024>(f, lds, &tb);
+    int64_t ps = block_excl_scan<1024>(sz, lds, &ts);
+    if (f) {
+      uint64_t key = t.uL[i] & 0xffffffffull;
+      for (int j = nleft - 1; j >= 0; j--) { uqi[(cb + pb) * rU + j] = (int)(key % A.nsec[j]); key /= A.nsec[j]; }
+      uqi[(cb + pb) * rU + nleft] = midnum[q];
+      uoff[cb + pb] = co + ps;
+    }
+    cb += tb;
+    co += ts;
+  }
+  if (threadIdx.x == 0) uoff[cb] = co;
+  __syncthreads();
+  cb = 0;
+  co = 0;
+  for (int base = 0; base < nuR; base += 1024) {
+    int i = base + threadIdx.x;
+    int64_t f = 0, sz = 0;
+    int q = -1;
+    if (i < nuR) {
+      const uint64_t qc = t.uR[i] >> 32;
+      int lo = 0, hi = nQ;
+      while (lo < hi) { int m = (lo + hi) >> 1; if (t.qc[m] < qc) lo = m + 1; else hi = m; }
+      q = lo;
+      f = newdim[q] > 0;
+      sz = f ? t.rsz[i] * newdim[q] : 0;
+    }
+    int64_t tb, ts;
+    int64_t pb = block_excl_scan<1024>(f, lds, &tb);
+    int64_t ps = block_excl_scan<1024>(sz, lds, &ts);
+    if (f) {
+      uint64_t key = t.uR[i] & 0xffffffffull;
+      vqi[(cb + pb) * rV] = midnum[q];
+      for (int j = A.rank - 1; j >= nleft; j--) {
+        vqi[(cb + pb) * rV + 1 + (j - nleft)] = (int)(key % A.nsec[j]);
+        key /= A.nsec[j];
+      }
+      voff[cb + pb] = co + ps;
+    }
+    cb += tb;
+    co += ts;
+  }
+  if (threadIdx.x == 0) voff[cb] = co;
+}
+
+// 6b. slice sector matrices into blocks.  U_q (rows x k, ld rows) ->
+// U blocks (lsize x k);  W_q (k x cols, ld k) -> V blocks (k x rsize).
+__global__ void k_svd_scatter(SvdTab t, int nQ, const int* newdim, const int* ublk0,
+                              const int* vblk0, const int64_t* uoffs, const int64_t* voffs,
+                              const double* Umat, const int64_t* umo, const double* Wmat,
+                              const int64_t* wmo, double* Ud, double* Vd) {
+  const int q = blockIdx.x;
+  const int k = newdim[q];
+  if (k == 0) return;
+  const int64_t rows = t.rows[q], cols = t.cols[q];
+  int ub = ublk0[q];
+  for (int i = t.qL0[q]; i < t.qL1[q]; i++, ub++) {
+    const int64_t ls = t.lsz[i], lo = t.loff[i];
+    double* dst = Ud + uoffs[ub];
+    for (int64_t e = threadIdx.x; e < ls * k; e += blockDim.x) {
+      const int64_t r = e % ls, c = e / ls;
+      dst[e] = Umat[umo[q] + lo + r + rows * c];
+    }
+  }
+  int vb = vblk0[q];
+  for (int i = t.qR0[q]; i < t.qR1[q]; i++, vb++) {
+    const int64_t rs = t.rsz[i], co = t.roff[i];
+    double* dst = Vd + voffs[vb];
+    for (int64_t e = threadIdx.x; e < rs * k; e += blockDim.x) {
+      const int64_t r = e % k, c = e / k;
+      dst[e] = Wmat[wmo[q] + r + (int64_t)k * (co + c)];
+    }
+  }
+  (void)cols;
+}
+
+// transpose-free copies for the unabsorbed factor: U_q = E (rows x k) for
+// MoveFromLeft; W_q = E^T (k x cols) for MoveFromRight
+__global__ void k_svd_transpose(int nQ, const int* nq, const int* newdim, const int64_t* selo,
+                                const double* E, const int64_t* wmo, double* W) {
+  const int q = blockIdx.x;
+  const int n = nq[q], k = newdim[q];
+  for (int64_t e = threadIdx.x; e < (int64_t)n * k; e += blockDim.x) {
+    const int64_t i = e % n, c = e / n;   // E(i, c)
+    W[wmo[q] + c + (int64_t)k * i] = E[selo[q] + e];
+  }
+}
+
+__global__ void k_svd_svals(int nQ, const int* newdim, const int64_t* soff, const double* svals,
+                            const int64_t* sout_off, double* S) {
+  const int q = blockIdx.x;
+  for (int i = threadIdx.x; i < newdim[q]; i += blockDim.x) S[sout_off[q] + i] = svals[soff[q] + i];
+}
+
+// grouped GEMM over sectors with host-built tasks
+struct HostGemm {
+  std::vector<GemmTask> tasks;
+  std::vector<CBlock> cblk;
+};
+
+static int run_host_gemm(const HostGemm& hg, const double* A, const double* B, double* C) {
+  if (hg.cblk.empty()) return 0;
+  ttg_plan_s p;
+  p.prec = TTG_FP64;
+  p.cfg = TILE_64x64;
+  std::vector<Tile> tiles;
+  for (size_t c = 0; c < hg.cblk.size(); c++) {
+    const CBlock& b = hg.cblk[c];
+    for (int m0 = 0; m0 < b.M; m0 += 64)
+      for (int n0 = 0; n0 < b.N; n0 += 64) tiles.push_back(Tile{(int)c, m0, n0, b.t1 - b.t0});
+  }
+  p.ntiles = (int)tiles.size();
+  p.ntasks = (int64_t)hg.tasks.size();
+  const GemmTask& g0 = hg.tasks[0];
+  p.a_mcontig = g0.sam0 == 1;
+  p.b_kcontig = g0.sbk0 == 1;
+  p.d_tasks = dalloc<GemmTask>(hg.tasks.size());
+  p.d_cblk = dalloc<CBlock>(hg.cblk.size());
+  p.d_tiles = dalloc<Tile>(tiles.size());
+  hipStream_t s = stream();
+  TTG_HIP(hipMemcpyAsync(p.d_tasks, hg.tasks.data(), sizeof(GemmTask) * hg.tasks.size(),
+                         hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(p.d_cblk, hg.cblk.data(), sizeof(CBlock) * hg.cblk.size(),
+                         hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(p.d_tiles, tiles.data(), sizeof(Tile) * tiles.size(),
+                         hipMemcpyHostToDevice, s));
+  TTG_TRY(gemm_launch(p, A, B, C, true, s));
+  TTG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
+  return 0;   // p's destructor releases the device arrays
+}
+
+static GemmTask gtask(int64_t a_off, int64_t b_off, int em0, int64_t sam0, int ek0, int64_t sak0,
+                      int64_t sbk0, int en0, int64_t sbn0, int K) {
+  GemmTask g{};
+  g.a_off = a_off; g.b_off = b_off;
+  g.em0 = em0; g.sam0 = sam0; g.sam1 = 0;
+  g.ek0 = ek0; g.sak0 = sak0; g.sak1 = 0; g.sbk0 = sbk0; g.sbk1 = 0;
+  g.en0 = en0; g.sbn0 = sbn0; g.sbn1 = 0;
+  g.K = K;
+  return g;
+}
+
+template <class T>
+static T* up(const std::vector<T>& v) {
+  T* d = dalloc<T>(std::max<size_t>(v.size(), 1));
+  if (!v.empty()) hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream());
+  return d;
+}
+
+}  // namespace ttg
 
 using namespace ttg;
 
 extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, int64_t K,
-                       ttg_tensor* U, ttg_tensor* S, ttg_tensor* V, double* trunc_err,
+                       ttg_tensor* Uout, ttg_tensor* Sout, ttg_tensor* Vout, double* trunc_err,
                        int64_t* kept) {
-  (void)A; (void)nleft; (void)direction; (void)cutoff; (void)K; (void)U; (void)S; (void)V;
-  (void)trunc_err; (void)kept;
-  set_error("ttg_svd: not built yet");
-  return 1;
+  TTG_REQUIRE(A && Uout && Sout && Vout, "svd: null argument");
+  const TMeta& m = *A->m;
+  TTG_REQUIRE(nleft >= 1 && nleft < m.rank, "svd: need at least one left and one right index");
+  TTG_REQUIRE(direction == TTG_MOVE_FROM_LEFT || direction == TTG_MOVE_FROM_RIGHT, "svd: direction");
+  double rl = 1, rr = 1;
+  for (int j = 0; j < m.rank; j++) (j < nleft ? rl : rr) *= m.nsec[j];
+  TTG_REQUIRE(rl < 4294967295.0 && rr < 4294967295.0, "svd: sector-tuple key exceeds 32 bits");
+  hipStream_t s = stream();
+  const int nb = m.nblocks;
+  TTG_REQUIRE(nb > 0, "svd: tensor without blocks");
+  // ---- 1. bookkeeping
+  uint64_t *kl = dalloc<uint64_t>(nb), *kr = dalloc<uint64_t>(nb);
+  uint64_t *sl = dalloc<uint64_t>(nb), *sr = dalloc<uint64_t>(nb);
+  int *ids = dalloc<int>(nb), *sid = dalloc<int>(nb);
+  SvdTab t{};
+  t.uL = dalloc<uint64_t>(nb); t.uR = dalloc<uint64_t>(nb);
+  t.lsz = dalloc<int64_t>(nb); t.loff = dalloc<int64_t>(nb);
+  t.rsz = dalloc<int64_t>(nb); t.roff = dalloc<int64_t>(nb);
+  t.qL0 = dalloc<int>(nb); t.qL1 = dalloc<int>(nb); t.qR0 = dalloc<int>(nb); t.qR1 = dalloc<int>(nb);
+  t.rows = dalloc<int64_t>(nb); t.cols = dalloc<int64_t>(nb);
+  t.qc = dalloc<uint32_t>(nb);
+  t.hdr = dalloc<int64_t>(4);
+  std::vector<void*> scratch{kl, kr, sl, sr, ids, sid, t.uL, t.uR, t.lsz, t.loff, t.rsz, t.roff,
+                             t.qL0, t.qL1, t.qR0, t.qR1, t.rows, t.cols, t.qc, t.hdr};
+  auto release = [&]() { for (void* p : scratch) dfree(p); };
+  const unsigned g1 = (nb + 255) / 256;
+  hipLaunchKernelGGL(k_svd_keys, dim3(g1), dim3(256), 0, s, m.view(), nleft, kl, kr, ids);
+  hipLaunchKernelGGL(k_sort_keys, dim3(g1), dim3(256), 0, s, kl, ids, nb, sl, sid);
+  hipLaunchKernelGGL(k_sort_keys, dim3(g1), dim3(256), 0, s, kr, ids, nb, sr, sid);
+  hipLaunchKernelGGL(k_svd_unique, dim3(1), dim3(1024), 0, s, m.view(), nleft, sl, sr, nb, t);
+  int64_t hdr[4] = {0, 0, 0, 0};
+  TTG_HIP(hipMemcpyAsync(hdr, t.hdr, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  const int nuL = (int)hdr[0], nuR = (int)hdr[1], nQ = (int)hdr[2];
+  std::vector<int64_t> rows(nQ), cols(nQ);
+  std::vector<uint32_t> qcs(nQ);
+  std::vector<int> qL0(nQ), qL1(nQ), qR0(nQ), qR1(nQ);
+  TTG_HIP(hipMemcpyAsync(rows.data(), t.rows, sizeof(int64_t) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(cols.data(), t.cols, sizeof(int64_t) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(qcs.data(), t.qc, sizeof(uint32_t) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(qL0.data(), t.qL0, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(qL1.data(), t.qL1, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(qR0.data(), t.qR0, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(qR1.data(), t.qR1, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  // ---- host layout of the per-sector work arrays
+  const bool fromLeft = direction == TTG_MOVE_FROM_LEFT;
+  std::vector<int64_t> moff(nQ + 1, 0), goff(nQ + 1, 0), poff(nQ + 1, 0), eoff(nQ + 1, 0);
+  std::vector<int> nq(nQ), kq(nQ);
+  int64_t maxpsize = 0;
+  for (int q = 0; q < nQ; q++) {
+    const int64_t n = fromLeft ? rows[q] : cols[q];
+    TTG_REQUIRE(n <= 1024, "svd: sector larger than 1024 (Jacobi pair table)");
+    nq[q] = (int)n;
+    int kd = (int)std::min(rows[q], cols[q]);
+    kq[q] = K > 0 ? (int)std::min<int64_t>(K, kd) : kd;   // rank-K SVD per sector
+    moff[q + 1] = moff[q] + rows[q] * cols[q];
+    goff[q + 1] = goff[q] + n * n;
+    const int64_t np = n + (n & 1);
+    poff[q + 1] = poff[q] + np * (np + 1) / 2;
+    eoff[q + 1] = eoff[q] + n;
+    maxpsize = std::max(maxpsize, np * (np + 1) / 2);
+  }
+  std::vector<int64_t> soff(nQ + 1, 0), selo(nQ + 1, 0);
+  for (int q = 0; q < nQ; q++) {
+    soff[q + 1] = soff[q] + kq[q];
+    selo[q + 1] = selo[q] + (int64_t)nq[q] * kq[q];
+  }
+  double* mat = dalloc<double>(moff[nQ]);
+  double* G = dalloc<double>(goff[nQ]);
+  double* Vev = dalloc<double>(goff[nQ]);
+  double* gs = dalloc<double>(poff[nQ]);
+  double* ev = dalloc<double>(eoff[nQ]);
+  double* Esel = dalloc<double>(selo[nQ]);
+  double* svals = dalloc<double>(soff[nQ]);
+  int* ssec = dalloc<int>(soff[nQ]);
+  int* order = dalloc<int>(soff[nQ]);
+  int* newdim = dalloc<int>(nQ);
+  double* tout = dalloc<double>(2);
+  int64_t* d_moff = up(moff);
+  int64_t* d_goff = up(goff);
+  int64_t* d_poff = up(poff);
+  int64_t* d_eoff = up(eoff);
+  int64_t* d_soff = up(soff);
+  int64_t* d_selo = up(selo);
+  int* d_nq = up(nq);
+  int* d_kq = up(kq);
+  for (void* p : std::initializer_list<void*>{mat, G, Vev, gs, ev, Esel, svals, ssec, order, newdim, tout,
+                                              d_moff, d_goff, d_poff, d_eoff, d_soff, d_selo, d_nq, d_kq})
+    scratch.push_back(p);
+  // ---- 2. gather
+  fill_f64(mat, moff[nQ], 0.0);
+  hipLaunchKernelGGL(k_svd_gather, dim3(std::min(nb, 4096)), dim3(256), 0, s, m.view(), nleft,
+                     A->d_data, t, nuL, nuR, nQ, d_moff, mat);
+  // ---- 3. Gram on MFMA
+  {
+    HostGemm hg;
+    for (int q = 0; q < nQ; q++) {
+      const int n = nq[q];
+      const int64_t r = rows[q], c = cols[q];
+      if (fromLeft)   // G = A A^T : A(m=i, k=j) = mat[i + r j]; B(k=j, n=i') = mat[i' + r j]
+        hg.tasks.push_back(gtask(moff[q], moff[q], (int)r, 1, (int)c, r, r, (int)r, 1, (int)c));
+      else            // G = A^T A : A(m=j, k=i) = mat[i + r j]; B(k=i, n=j') = mat[i + r j']
+        hg.tasks.push_back(gtask(moff[q], moff[q], (int)c, r, (int)r, 1, 1, (int)c, r, (int)r));
+      hg.cblk.push_back(CBlock{goff[q], n, n, q, q + 1});
+    }
+    if (run_host_gemm(hg, mat, mat, G)) { release(); return 1; }
+  }
+  // ---- 4. eigensolver
+  const size_t lds_bytes = sizeof(double) * (size_t)std::min<int64_t>(maxpsize, kLdsDoubles);
+  TTG_HIP(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(sizeof(double) * kLdsDoubles)));
+  hipLaunchKernelGGL(k_jacobi, dim3(nQ), dim3(kJacThreads), lds_bytes, s, d_nq, d_goff, G, gs,
+                     d_poff, Vev, ev, d_eoff, 40);
+  TTG_HIP(hipGetLastError());
+  // ---- 5. selection and truncation
+  hipLaunchKernelGGL(k_svd_select, dim3(nQ), dim3(256), 0, s, d_nq, d_kq, d_eoff, ev, d_goff, Vev,
+                     d_soff, svals, ssec, d_selo, Esel);
+  TTG_HIP(hipMemsetAsync(newdim, 0, sizeof(int) * nQ, s));
+  hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(256), 0, s, (int)soff[nQ], svals, ssec, order,
+                     cutoff, K, newdim, tout);
+  std::vector<int> nd(nQ);
+  double th[2];
+  TTG_HIP(hipMemcpyAsync(nd.data(), newdim, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipMemcpyAsync(th, tout, sizeof(th), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  if (trunc_err) *trunc_err = th[0];
+  if (kept) *kept = (int64_t)th[1];
+  // ---- 6. outputs: index tables (left..., mid) (mid, right...) (mid)
+  std::vector<int> midnum(nQ, -1);
+  std::vector<int> mq, md;
+  int nm = 0;
+  for (int q = 0; q < nQ; q++)
+    if (nd[q] > 0) {
+      midnum[q] = nm++;
+      mq.push_back((int)(qcs[q] >> 16) - 32768);
+      mq.push_back((int)(qcs[q] & 0xffff) - 32768);
+      md.push_back(nd[q]);
+    }
+  TTG_REQUIRE(nm > 0, "svd: no singular value kept");
+  std::vector<std::pair<const TMeta*, int>> lsrc, rsrc;
+  for (int j = 0; j < nleft; j++) lsrc.push_back({&m, j});
+  for (int j = nleft; j < m.rank; j++) rsrc.push_back({&m, j});
+  MetaPtr mid;
+  int arrOut = 1, arrIn = 0;
+  TTG_TRY(meta_shell(1, &arrOut, &nm, mq.data(), md.data(), &mid));
+  MetaPtr midIn;
+  TTG_TRY(meta_shell(1, &arrIn, &nm, mq.data(), md.data(), &midIn));
+  auto concat = [](const std::vector<std::pair<const TMeta*, int>>& a) { return a; };
+  std::vector<std::pair<const TMeta*, int>> usrc = concat(lsrc), vsrc{{midIn.get(), 0}};
+  usrc.push_back({mid.get(), 0});
+  vsrc.insert(vsrc.end(), rsrc.begin(), rsrc.end());
+  MetaPtr Um = meta_from_indices(usrc), Vm = meta_from_indices(vsrc);
+  TTG_REQUIRE(Um && Vm, "svd: meta allocation failed");
+  int nUb = 0, nVb = 0;
+  int64_t Utot = 0, Vtot = 0, Stot = 0;
+  std::vector<int> ublk0(nQ, 0), vblk0(nQ, 0);
+  std::vector<int64_t> sout(nQ + 1, 0);
+  for (int q = 0; q < nQ; q++) {
+    ublk0[q] = nUb;
+    vblk0[q] = nVb;
+    sout[q + 1] = sout[q] + nd[q];
+    if (nd[q] == 0) continue;
+    nUb += qL1[q] - qL0[q];
+    nVb += qR1[q] - qR0[q];
+    Utot += rows[q] * nd[q];
+    Vtot += cols[q] * nd[q];
+    Stot += nd[q];
+  }
+  TTG_TRY(meta_alloc_blocks(*Um, nUb, (size_t)Utot));
+  TTG_TRY(meta_alloc_blocks(*Vm, nVb, (size_t)Vtot));
+  TTG_TRY(meta_alloc_blocks(*mid, nm, (size_t)Stot));
+  {
+    std::vector<int> sqi(nm);
+    std::iota(sqi.begin(), sqi.end(), 0);
+    std::vector<int64_t> soffv(nm + 1, 0);
+    for (int q = 0, k = 0; q < nQ; q++)
+      if (nd[q] > 0) { soffv[k + 1] = soffv[k] + nd[q]; k++; }
+    TTG_HIP(hipMemcpyAsync(mid->d_qi, sqi.data(), sizeof(int) * nm, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(mid->d_off, soffv.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, s));
+    TTG_HIP(hipStreamSynchronize(s));
+  }
+  int* d_mid = up(midnum);
+  int* d_ub0 = up(ublk0);
+  int* d_vb0 = up(vblk0);
+  int64_t* d_sout = up(sout);
+  scratch.push_back(d_mid); scratch.push_back(d_ub0); scratch.push_back(d_vb0); scratch.push_back(d_sout);
+  hipLaunchKernelGGL(k_svd_emit, dim3(1), dim3(1024), 0, s, m.view(), nleft, t, nQ, newdim, d_mid,
+                     Um->d_qi, Um->d_off, Vm->d_qi, Vm->d_off);
+  ttg_tensor U, V, Sv;
+  TTG_TRY(tensor_from_meta(Um, false, &U));
+  TTG_TRY(tensor_from_meta(Vm, false, &V));
+  TTG_TRY(tensor_from_meta(mid, false, &Sv));
+  hipLaunchKernelGGL(k_svd_svals, dim3(nQ), dim3(64), 0, s, nQ, newdim, d_soff, svals, d_sout, Sv->d_data);
+  // absorbed factor: MoveFromLeft  W = E^T A (k x cols), U = E (rows x k)
+  //                  MoveFromRight U = A E (rows x k),   W = E^T (k x cols)
+  std::vector<int64_t> umo(nQ + 1, 0), wmo(nQ + 1, 0);
+  for (int q = 0; q < nQ; q++) {
+    umo[q + 1] = umo[q] + rows[q] * nd[q];
+    wmo[q + 1] = wmo[q] + (int64_t)nd[q] * cols[q];
+  }
+  double* Umat = dalloc<double>(umo[nQ]);
+  double* Wmat = dalloc<double>(wmo[nQ]);
+  int64_t* d_umo = up(umo);
+  int64_t* d_wmo = up(wmo);
+  scratch.push_back(Umat); scratch.push_back(Wmat); scratch.push_back(d_umo); scratch.push_back(d_wmo);
+  {
+    HostGemm hg;
+    for (int q = 0; q < nQ; q++) {
+      if (nd[q] == 0) continue;
+      const int64_t r = rows[q], c = cols[q], n = nq[q];
+      const int k = nd[q];
+      if (fromLeft)   // W(i, j) = sum_a E(a, i) A(a, j): A-op E^T (m=i,k=a) = E[a + n i]
+        hg.tasks.push_back(gtask(selo[q], moff[q], k, n, (int)r, 1, 1, (int)c, r, (int)r));
+      else            // U(i, j) = sum_a A(i, a) E(a, j)
+        hg.tasks.push_back(gtask(moff[q], selo[q], (int)r, 1, (int)c, r, 1, k, n, (int)c));
+      const int tix = (int)hg.tasks.size() - 1;
+      hg.cblk.push_back(fromLeft ? CBlock{wmo[q], k, (int)c, tix, tix + 1}
+                                 : CBlock{umo[q], (int)r, k, tix, tix + 1});
+    }
+    int rc = fromLeft ? run_host_gemm(hg, Esel, mat, Wmat) : run_host_gemm(hg, mat, Esel, Umat);
+    if (rc) { release(); return 1; }
+  }
+  if (fromLeft) {   // U_q = E restricted to k columns: Esel already (n = rows) x k, ld rows
+    for (int q = 0; q < nQ; q++)
+      if (nd[q] > 0)
+        TTG_HIP(hipMemcpyAsync(Umat + umo[q], Esel + selo[q], sizeof(double) * rows[q] * nd[q],
+                               hipMemcpyDeviceToDevice, s));
+  } else {
+    hipLaunchKernelGGL(k_svd_transpose, dim3(nQ), dim3(256), 0, s, nQ, d_nq, newdim, d_selo, Esel,
+                       d_wmo, Wmat);
+  }
+  hipLaunchKernelGGL(k_svd_scatter, dim3(nQ), dim3(256), 0, s, t, nQ, newdim, d_ub0, d_vb0, Um->d_off,
+                     Vm->d_off, Umat, d_umo, Wmat, d_wmo, U->d_data, V->d_data);
+  TTG_HIP(hipStreamSynchronize(s));
+  release();
+  *Uout = U;
+  *Vout = V;
+  *Sout = Sv;
+  return 0;
 }

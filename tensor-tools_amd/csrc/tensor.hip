@@ -409,6 +409,29 @@ int ttg_tensor_index_download(ttg_tensor t, int j, int* nsec, int* qn, int* qdim
   return 0;
 }
 
+int ttg_tensor_dag(ttg_tensor t) {
+  TTG_REQUIRE(t, "null tensor");
+  const TMeta& o = *t->m;
+  // new structure object: same sectors and blocks, flipped arrows
+  auto m = std::make_shared<TMeta>();
+  m->rank = o.rank;
+  m->h_idx = o.h_idx;
+  for (int j = 0; j <= MAXR; j++) m->idx_off[j] = o.idx_off[j];
+  for (int j = 0; j < o.rank; j++) {
+    m->arrow[j] = 1 - o.arrow[j];
+    m->nsec[j] = o.nsec[j];
+  }
+  TTG_TRY(upload_idx(*m));
+  TTG_TRY(meta_alloc_blocks(*m, o.nblocks, o.total));
+  if (o.nblocks)
+    TTG_HIP(hipMemcpyAsync(m->d_qi, o.d_qi, sizeof(int) * (size_t)o.nblocks * o.rank,
+                           hipMemcpyDeviceToDevice, stream()));
+  TTG_HIP(hipMemcpyAsync(m->d_off, o.d_off, sizeof(int64_t) * ((size_t)o.nblocks + 1),
+                         hipMemcpyDeviceToDevice, stream()));
+  t->m = m;
+  return 0;
+}
+
 int ttg_same_blocks(ttg_tensor a, ttg_tensor b, int* same) {
   TTG_REQUIRE(a && b && same, "same_blocks: null argument");
   *same = same_structure(a, b) ? 1 : 0;

@@ -1,0 +1,666 @@
+// mps.cpp -- qMPS/qMPO containers, spin-1/2 Heisenberg MPO builders, SVD
+// wrappers and the two-site DMRG driver of the reference, over device blocks.
+//   mps/qtt.cpp:60-78, 205-311, 821-994; models/hams/Heisenberg.cpp:136-180,
+//   333-635; models/hams/AutoMPO.{h,cpp}; models/lattice/square.h;
+//   qtensor/qtensor_op.cpp:1097-1544; algos/dmrg/dmrg.cpp:56-84, 300-372,
+//   491-509, 760-813.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <set>
+#include <unordered_map>
+
+#include "tt_gpu.hpp"
+
+namespace ttgpu {
+
+// ---------------------------------------------------------------- host blocks
+// Block enumeration on the host for the input builders only (MPO / product
+// state coefficients are written by key before upload).  Same order as
+// qtensor::initBlock (qtensor.cpp:302-345).
+namespace {
+struct HostBlocks {
+  std::vector<std::vector<unsigned>> qi;
+  std::vector<size_t> off;
+  std::unordered_map<std::string, unsigned> by_key;
+  std::vector<double> data;
+};
+std::string key_of(const std::vector<QN_t>& q) {
+  std::string s;
+  for (auto& x : q) s += to_string(x) + " ";
+  return s;
+}
+HostBlocks host_init_blocks(const std::vector<qtensor_index>& idx) {
+  HostBlocks hb;
+  unsigned rank = (unsigned)idx.size();
+  std::vector<unsigned> stride(1, 1);
+  for (unsigned i = 0; i < rank; i++) stride.push_back(idx[i].size() * stride[i]);
+  size_t total = 0;
+  for (unsigned i = 0; i < stride.back(); i++) {
+    std::vector<unsigned> pos(rank);
+    std::vector<QN_t> qns(rank);
+    size_t size = 1;
+    QN_t tot;
+    for (unsigned j = 0; j < rank; j++) {
+      pos[j] = (i / stride[j]) % idx[j].size();
+      qns[j] = idx[j].qn(pos[j]);
+      size *= idx[j].qdim(pos[j]);
+      if (idx[j].arrow() == Inward) tot += qns[j]; else tot -= qns[j];
+    }
+    if (tot == QN_t()) {
+      hb.by_key[key_of(qns)] = (unsigned)hb.qi.size();
+      hb.qi.push_back(pos);
+      hb.off.push_back(total);
+      total += size;
+    }
+  }
+  hb.off.push_back(total);
+  hb.data.assign(total, 0.0);
+  return hb;
+}
+std::string link_name(unsigned id, unsigned i) { return "ID" + std::to_string(id) + "Link" + std::to_string(i); }
+}  // namespace
+
+// ---------------------------------------------------------------- sites
+// models/sites/spinhalf.cpp:23-46 (Dn = 0, Up = 1)
+static double spinhalf_op(unsigned bra, const std::string& op, unsigned ket) {
+  if (op == "Id" || op == "I") return ket == bra ? 1 : 0;
+  if (op == "Sz") return ket != bra ? 0 : (ket == 0 ? -0.5 : 0.5);
+  if (op == "S+") return (ket == 0 && bra == 1) ? 1 : 0;
+  if (op == "S-") return (ket == 1 && bra == 0) ? 1 : 0;
+  fail("spin-1/2 operator lookup");
+}
+static QN_t spinhalf_div(const std::string& op) {   // spinhalf.cpp:95-106
+  if (op == "S+") return QN_t(2, 0);
+  if (op == "S-") return QN_t(-2, 0);
+  return QN_t();
+}
+
+// ---------------------------------------------------------------- MPO builders
+namespace {
+struct MpoBuild {
+  unsigned N, id;
+  std::vector<QN_t> phy_qn{QN_t(-1, 0), QN_t(1, 0)};
+  std::vector<std::vector<qtensor_index>> idx;
+  std::vector<HostBlocks> hb;
+  // Heisenberg.cpp:136-180: write val*<b|op|k> into block (Qi, k, b, Qo) at
+  // offset s0*r + s3*c (s3*c on site 0); a write overwrites
+  void add_operator(unsigned site, unsigned r, unsigned c, const std::string& op, double val,
+                    QN_t Qi, QN_t Qo) {
+    bool added = false;
+    for (unsigned k = 0; k < 2; k++)
+      for (unsigned b = 0; b < 2; b++) {
+        std::string s = to_string(Qi) + " " + to_string(phy_qn[k]) + " " + to_string(phy_qn[b]) +
+                        " " + to_string(Qo) + " ";
+        auto it = hb[site].by_key.find(s);
+        if (it == hb[site].by_key.end()) continue;
+        added = true;
+        unsigned bp = it->second;
+        const auto& q = hb[site].qi[bp];
+        size_t s1 = idx[site][0].qdim(q[0]);
+        size_t s2 = s1 * idx[site][1].qdim(q[1]);
+        size_t s3 = s2 * idx[site][2].qdim(q[2]);
+        size_t off = (site == 0) ? s3 * c : r + s3 * c;
+        if (off >= hb[site].off[bp + 1] - hb[site].off[bp]) fail("MPO coefficient outside its block");
+        hb[site].data[hb[site].off[bp] + off] = val * spinhalf_op(b, op, k);
+      }
+    if (!added) fail("MPO operator has no block");
+  }
+  qMPO<double> finish() {
+    qMPO<double> H;
+    H._id = id; H.length = N; H.phy_dim = 2; H.phy_qn = phy_qn;
+    H.bond_dims.assign(N + 1, 1);
+    for (unsigned i = 0; i < N; i++) {
+      qtensor<double> t(idx[i]);
+      t.from_host(hb[i].qi, hb[i].data);
+      H.A.push_back(std::move(t));
+    }
+    H.tensors_allocated = true;
+    return H;
+  }
+  std::vector<qtensor_index> site_indices(unsigned i) {
+    return {qtensor_index(Inward, link_name(id, i), Link, 0),
+            qtensor_index(Outward, "Site" + std::to_string(i), Site, 0),
+            qtensor_index(Inward, "Site" + std::to_string(i), Site, 1),
+            qtensor_index(Outward, link_name(id, i + 1), Link, 0)};
+  }
+};
+}  // namespace
+
+// Heisenberg.cpp:333-402
+qMPO<double> heisenberg_mpo_nn(unsigned N, unsigned id) {
+  MpoBuild b{N, id};
+  for (unsigned i = 0; i < N; i++) {
+    auto ix = b.site_indices(i);
+    if (i == 0) ix[0].addQN(QN_t(), 1);
+    else { ix[0].addQN(QN_t(), 3); ix[0].addQN(QN_t(-2, 0), 1); ix[0].addQN(QN_t(2, 0), 1); }
+    for (auto& q : b.phy_qn) { ix[1].addQN(q, 1); ix[2].addQN(q, 1); }
+    if (i == N - 1) ix[3].addQN(QN_t(), 1);
+    else { ix[3].addQN(QN_t(), 3); ix[3].addQN(QN_t(-2, 0), 1); ix[3].addQN(QN_t(2, 0), 1); }
+    b.hb.push_back(host_init_blocks(ix));
+    b.idx.push_back(ix);
+  }
+  for (unsigned s = 0; s < N; s++) {
+    if (s > 0) b.add_operator(s, 0, 0, "Id", 1.0, QN_t(), QN_t());
+    if (s < N - 1) b.add_operator(s, 1, 1, "Id", 1.0, QN_t(), QN_t());
+    if (s > 0) b.add_operator(s, 2, 0, "Sz", 1.0, QN_t(), QN_t());
+    if (s < N - 1) b.add_operator(s, 1, 2, "Sz", 1.0, QN_t(), QN_t());
+    if (s > 0) b.add_operator(s, 0, 0, "S+", 1.0, QN_t(-2, 0), QN_t());
+    if (s > 0) b.add_operator(s, 0, 0, "S-", 1.0, QN_t(2, 0), QN_t());
+    if (s < N - 1) b.add_operator(s, 1, 0, "S-", 0.5, QN_t(), QN_t(-2, 0));
+    if (s < N - 1) b.add_operator(s, 1, 0, "S+", 0.5, QN_t(), QN_t(2, 0));
+  }
+  return b.finish();
+}
+
+// AutoMPO.{h,cpp}: SiteTerm order (site, then op); HTerm set ordered by
+// LessNoCoef with duplicates summed.  Heisenberg.cpp:416-635: link bases
+// (IL, HL and every first operator of a term crossing the link), the
+// zero-QN-first std::sort of each basis, sector grouping, basisToIndex
+// numbering and the start / continue / end operator placements.
+namespace {
+struct SiteTerm {
+  std::string op; int i = -1;
+  bool operator==(const SiteTerm& o) const { return op == o.op && i == o.i; }
+  bool operator!=(const SiteTerm& o) const { return !(*this == o); }
+  bool operator<(const SiteTerm& o) const { return i != o.i ? i < o.i : op < o.op; }
+};
+struct HTerm {
+  double coef = 1;
+  std::vector<SiteTerm> ops;
+  const SiteTerm& first() const { return ops.front(); }
+  const SiteTerm& last() const { return ops.back(); }
+};
+struct LessNoCoef {
+  bool operator()(const HTerm& a, const HTerm& b) const {
+    if (a.ops.size() != b.ops.size()) return a.ops.size() < b.ops.size();
+    for (size_t j = 0; j < a.ops.size(); j++)
+      if (a.ops[j] != b.ops[j]) return a.ops[j] < b.ops[j];
+    return false;
+  }
+};
+struct SiteQN { SiteTerm st; QN_t q; };
+unsigned basis_to_index(QN_t q, std::map<QN_t, unsigned>& maxidx,
+                        std::unordered_map<unsigned, unsigned>& rc, unsigned i) {
+  if (!maxidx.count(q)) maxidx[q] = 1;
+  if (!rc.count(i)) rc[i] = maxidx[q]++;
+  return rc[i] - 1;
+}
+}  // namespace
+
+qMPO<double> autompo_mpo(unsigned N, unsigned id, const std::vector<SpinTerm>& in_terms) {
+  std::set<HTerm, LessNoCoef> terms;
+  for (auto& tm : in_terms) {
+    HTerm h;
+    for (auto& op : {std::make_pair(tm.op1, tm.i1), std::make_pair(tm.op2, tm.i2)}) {
+      auto it = h.ops.begin();
+      while (it != h.ops.end() && it->i <= op.second) ++it;
+      h.ops.insert(it, SiteTerm{op.first, op.second});
+    }
+    h.coef = tm.coef;
+    if (h.coef == 0.0) continue;
+    auto it = terms.find(h);
+    if (it == terms.end()) terms.insert(h);
+    else { HTerm nt = h; nt.coef += it->coef; terms.erase(it); terms.insert(nt); }
+  }
+  MpoBuild b{N, id};
+  const SiteTerm IL{"IL", 0}, HL{"HL", 0};
+  std::vector<std::vector<SiteQN>> basis(N + 1);
+  for (unsigned n = 0; n < N; n++) basis[n].push_back({IL, QN_t()});
+  for (unsigned n = 1; n <= N; n++) basis[n].push_back({HL, QN_t()});
+  for (auto& ht : terms)
+    for (int n = ht.first().i + 1; n <= ht.last().i + 1; ++n) {
+      auto& bn = basis.at(n);
+      bool has = false;
+      for (auto& sq : bn) if (sq.st == ht.first()) { has = true; break; }
+      if (!has) bn.push_back({ht.first(), -spinhalf_div(ht.first().op)});
+    }
+  const QN_t Zero;
+  auto qn_comp = [&Zero](const SiteQN& a, const SiteQN& c) {
+    if (a.q == Zero && c.q != Zero) return true;
+    if (c.q == Zero && a.q != Zero) return false;
+    return a.q < c.q;
+  };
+  for (auto& bn : basis) std::sort(bn.begin(), bn.end(), qn_comp);
+  std::vector<std::vector<std::pair<QN_t, unsigned>>> links(N + 1);
+  for (unsigned n = 0; n <= N; n++) {
+    QN_t cq = basis[n].front().q;
+    unsigned cm = 0;
+    for (auto& sq : basis[n]) {
+      if (sq.q == cq) ++cm;
+      else { links[n].push_back({cq, cm}); cq = sq.q; cm = 1; }
+    }
+    links[n].push_back({cq, cm});
+  }
+  std::vector<std::vector<HTerm>> ht_by_n(N + 1);
+  for (auto& ht : terms)
+    for (auto& st : ht.ops) ht_by_n.at(st.i + 1).push_back(ht);
+  std::vector<std::map<QN_t, unsigned>> maxidx(N + 1);
+  std::vector<std::unordered_map<unsigned, unsigned>> rcidx(N + 1);
+  for (unsigned i = 0; i < N; i++) {
+    auto ix = b.site_indices(i);
+    for (auto& rq : links[i]) ix[0].addQN(rq.first, rq.second);
+    if (i != N - 1) for (auto& cq : links[i + 1]) ix[3].addQN(cq.first, cq.second);
+    else ix[3].addQN(QN_t(), 1);
+    for (auto& q : b.phy_qn) { ix[1].addQN(q, 1); ix[2].addQN(q, 1); }
+    b.hb.push_back(host_init_blocks(ix));
+    b.idx.push_back(ix);
+    auto& bn1 = basis[i];
+    auto& bn = basis[i + 1];
+    unsigned c_max = (i == N - 1) ? 1 : (unsigned)bn.size();
+    for (unsigned r_ = 0; r_ < bn1.size(); r_++)
+      for (unsigned c_ = 0; c_ < c_max; c_++) {
+        const SiteTerm& rst = bn1[r_].st;
+        const SiteTerm& cst = bn[c_].st;
+        QN_t qr = -bn1[r_].q, qc = -bn[c_].q;
+        unsigned r = (i == 0) ? 1 : r_, c = c_;
+        auto R_ = [&]() { return basis_to_index(-qr, maxidx[i], rcidx[i], r); };
+        auto C_ = [&]() { return basis_to_index(qc, maxidx[i + 1], rcidx[i + 1], c); };
+        if (cst.i == (int)i && rst == IL && cst.op != "HL" && cst.op != "IL") {
+          unsigned tr = R_(), tc = C_();
+          if (i == 0) tr = 1;
+          b.add_operator(i, tr, tc, cst.op, 1.0, qr, qc);
+        }
+        if (cst == rst) {
+          unsigned tr = R_(), tc = C_();
+          b.add_operator(i, tr, tc, "Id", 1.0, qr, qc);
+        }
+        if (cst == HL)
+          for (auto& ht : ht_by_n[i + 1])
+            if (rst == ht.first() && ht.last().i == (int)i) {
+              unsigned tr = R_(), tc = C_();
+              b.add_operator(i, tr, tc, ht.last().op, ht.coef, qr, qc);
+            }
+        if (rst == IL && cst == HL)
+          for (auto& ht : ht_by_n[i + 1])
+            if (ht.first().i == ht.last().i) {
+              unsigned tr = R_(), tc = C_();
+              b.add_operator(i, tr, tc, ht.first().op, ht.coef, qr, qc);
+            }
+      }
+  }
+  return b.finish();
+}
+
+// models/lattice/square.h:25-108 (0-based sites)
+std::vector<std::pair<int, int>> square_lattice(int Nx, int Ny, bool yperiodic,
+                                                std::vector<int>* type, bool nnn) {
+  yperiodic = yperiodic && (Ny > 2);
+  std::vector<std::pair<int, int>> out;
+  auto add = [&](int a, int c, int t) { out.push_back({a - 1, c - 1}); if (type) type->push_back(t); };
+  for (int n = 1; n <= Nx * Ny; ++n) {
+    int x = (n - 1) / Ny + 1, y = (n - 1) % Ny + 1;
+    if (x < Nx) add(n, n + Ny, 1);
+    if (Ny > 1) {
+      if (y < Ny) add(n, n + 1, 1);
+      if (yperiodic && y == 1) add(n, n + Ny - 1, 1);
+    }
+    if (nnn && x < Nx && Ny > 1) {
+      if (y < Ny) add(n, n + Ny + 1, 2);
+      if (y > 1) add(n, n + Ny - 1, 2);
+      if (yperiodic && y == Ny) add(n, n + 1, 2);
+      if (yperiodic && y == 1) add(n, n + 2 * Ny - 1, 2);
+    }
+  }
+  return out;
+}
+
+std::vector<SpinTerm> heisenberg_terms(const std::vector<std::pair<int, int>>& bonds,
+                                       const std::vector<double>& J) {
+  std::vector<SpinTerm> t;
+  for (size_t k = 0; k < bonds.size(); k++) {
+    double j = J.size() == bonds.size() ? J[k] : J.at(0);
+    t.push_back({j, "Sz", bonds[k].first, "Sz", bonds[k].second});
+    t.push_back({0.5 * j, "S+", bonds[k].first, "S-", bonds[k].second});
+    t.push_back({0.5 * j, "S-", bonds[k].first, "S+", bonds[k].second});
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------- MPS
+// qtt.cpp:205-311 -- link QN sets (forward reachable, backward pruned),
+// all legal blocks, the product-state path set to 1
+qMPS<double> product_state_mps(unsigned id, const std::vector<unsigned>& state) {
+  qMPS<double> p;
+  const unsigned L = (unsigned)state.size();
+  p._id = id; p.length = L; p.phy_dim = 2; p.phy_qn = {QN_t(-1, 0), QN_t(1, 0)};
+  p.bond_dims.assign(L + 1, 1);
+  for (unsigned s : state) p.totalQ += p.phy_qn.at(s);
+  std::vector<std::set<QN_t>> LQ(L + 1);
+  LQ[0].insert(QN_t());
+  LQ[L].insert(p.totalQ);
+  for (unsigned j = 1; j < L; j++)
+    for (auto& q : LQ[j - 1])
+      for (auto& ph : p.phy_qn) LQ[j].insert(q + ph);
+  for (unsigned j = L - 1; j > 0; j--) {
+    std::vector<QN_t> rem;
+    for (auto& q : LQ[j]) {
+      bool ok = false;
+      for (auto& ph : p.phy_qn) if (LQ[j + 1].count(q + ph)) { ok = true; break; }
+      if (!ok) rem.push_back(q);
+    }
+    for (auto& q : rem) LQ[j].erase(q);
+  }
+  QN_t last;
+  for (unsigned i = 0; i < L; i++) {
+    std::vector<qtensor_index> ix{qtensor_index(Inward, link_name(id, i), Link, 0),
+                                  qtensor_index(Inward, "Site" + std::to_string(i), Site, 0),
+                                  qtensor_index(Outward, link_name(id, i + 1), Link, 0)};
+    for (auto& q : LQ[i]) ix[0].addQN(q, 1);
+    for (auto& q : p.phy_qn) ix[1].addQN(q, 1);
+    for (auto& q : LQ[i + 1]) ix[2].addQN(q, 1);
+    HostBlocks hb = host_init_blocks(ix);
+    std::string key = to_string(last) + " " + to_string(p.phy_qn[state[i]]) + " ";
+    last += p.phy_qn[state[i]];
+    key += to_string(last) + " ";
+    auto it = hb.by_key.find(key);
+    if (it == hb.by_key.end()) fail("product-state block missing");
+    hb.data[hb.off[it->second]] = 1.0;
+    qtensor<double> t(ix);
+    t.from_host(hb.qi, hb.data);
+    p.A.push_back(std::move(t));
+  }
+  p.center = (int)L - 1;
+  p.tensors_allocated = true;
+  return p;
+}
+
+// ---------------------------------------------------------------- SVD
+// qtensor_op.cpp:1097-1453 on the device: the left indices must be A's
+// leading indices in order (true for every caller in the DMRG flow).
+double svd(qtensor<double>& A, std::vector<qtensor_index>& left, std::vector<qtensor_index>& right,
+           qtensor<double>& U, qtensor<double>& V, qtensor<double>& S, int direction, double cutoff,
+           unsigned K, unsigned* kept) {
+  if (left.size() + right.size() != A.rank) fail("svd: index count");
+  for (size_t j = 0; j < left.size(); j++)
+    if (!(A.idx_set[j] == left[j])) fail("svd: left indices must lead A's index list");
+  for (size_t j = 0; j < right.size(); j++)
+    if (!(A.idx_set[left.size() + j] == right[j])) fail("svd: right indices must follow the left ones");
+  ttg_tensor u, s, v;
+  double trunc = 0;
+  int64_t kp = 0;
+  check(ttg_svd(A.dev.get(), (int)left.size(), direction, cutoff, (int64_t)K, &u, &s, &v, &trunc, &kp),
+        "svd");
+  // midCut: Outward, sectors of the kept singular values (qtensor_op.cpp:1303-1306)
+  int nsec = 0;
+  check(ttg_tensor_index_download(s, 0, &nsec, nullptr, nullptr), "svd mid");
+  std::vector<int> qn(2 * std::max(nsec, 1)), qd(std::max(nsec, 1));
+  check(ttg_tensor_index_download(s, 0, &nsec, qn.data(), qd.data()), "svd mid");
+  qtensor_index midCut(Outward, "svdmid", Link, 0);
+  for (int k = 0; k < nsec; k++) midCut.addQN(QN_t(qn[2 * k], qn[2 * k + 1]), (unsigned)qd[k]);
+  std::vector<qtensor_index> uidx(left);
+  uidx.push_back(midCut);
+  qtensor_index md = midCut;
+  md.dag();
+  std::vector<qtensor_index> vidx{md};
+  vidx.insert(vidx.end(), right.begin(), right.end());
+  U = qtensor<double>(uidx); U.dev = adopt(u); U._initted = true;
+  V = qtensor<double>(vidx); V.dev = adopt(v); V._initted = true;
+  S = qtensor<double>(std::vector<qtensor_index>{midCut}); S.dev = adopt(s); S._initted = true;
+  if (kept) *kept = (unsigned)kp;
+  return trunc;
+}
+
+// qtensor_op.cpp:1506-1544
+double svd_bond(qtensor<double>& combined, qtensor<double>& A_left, qtensor<double>& A_right,
+                qtensor_index& mid, qtensor<double>& S, int direction, double cutoff,
+                long unsigned K, unsigned* kept) {
+  qtensor<double> U, V;
+  std::vector<qtensor_index> left, right;
+  qtensor_index dag_mid(mid);
+  dag_mid.dag();
+  for (auto& ix : A_left.idx_set) if (ix.tag() != mid.tag()) left.push_back(ix);
+  for (auto& ix : A_right.idx_set) if (ix.tag() != dag_mid.tag()) right.push_back(ix);
+  double tr = svd(combined, left, right, U, V, S, direction, cutoff, (unsigned)K, kept);
+  U.idx_set.back().rename(mid.name());
+  U.idx_set.back().retype(mid.type());
+  V.idx_set[0].rename(dag_mid.name());
+  V.idx_set[0].retype(dag_mid.type());
+  if (U.idx_set.back().level() != mid.level())
+    U.idx_set.back().prime((int)mid.level() - (int)U.idx_set.back().level());
+  if (V.idx_set[0].level() != dag_mid.level())
+    V.idx_set[0].prime((int)dag_mid.level() - (int)V.idx_set[0].level());
+  A_right = std::move(V);
+  A_left = std::move(U);
+  return tr;
+}
+
+// qtensor.h:166-173: -sum s^2 log s^2 over s > cutoff
+double calcEntropy(qtensor<double>& S, double cutoff) {
+  std::vector<double> s = S.to_host();
+  double e = 0;
+  for (double v : s) if (v > cutoff) e -= v * v * std::log(v * v);
+  return e;
+}
+
+// ---------------------------------------------------------------- canonical form
+static void rename_like(qtensor_index& ix, const qtensor_index& mid) {
+  ix.rename(mid.name());
+  ix.prime((int)mid.level() - (int)ix.level());
+}
+
+template <unsigned N>
+void qTensorTrain<double, N>::rc() {   // qtt.cpp:821-850
+  qtensor<double> U, V, S;
+  for (unsigned i = length - 1; i > 0; i--) {
+    std::vector<qtensor_index> left, right;
+    qtensor_index mid;
+    std::string LN = link_name(_id, i);
+    for (auto& ix : A[i].idx_set) {
+      if (ix.name() == LN) { left.push_back(ix); mid = ix; } else right.push_back(ix);
+    }
+    svd(A[i], left, right, U, V, S, MoveFromRight);
+    bond_dims[i] = (unsigned)S.total_size();
+    A[i] = V;
+    rename_like(A[i].idx_set[0], mid);
+    A[i - 1] = A[i - 1] * U;
+    rename_like(A[i - 1].idx_set.back(), mid);
+  }
+  center = 0;
+}
+
+template <unsigned N>
+void qTensorTrain<double, N>::lc() {   // qtt.cpp:857-886
+  qtensor<double> U, V, S;
+  for (unsigned i = 0; i + 1 < length; i++) {
+    std::vector<qtensor_index> left, right;
+    qtensor_index mid;
+    std::string LN = link_name(_id, i + 1);
+    for (auto& ix : A[i].idx_set) {
+      if (ix.name() == LN) { right.push_back(ix); mid = ix; } else left.push_back(ix);
+    }
+    svd(A[i], left, right, U, V, S, MoveFromLeft);
+    bond_dims[i + 1] = (unsigned)S.total_size();
+    A[i] = U;
+    rename_like(A[i].idx_set.back(), mid);
+    A[i + 1] = V * A[i + 1];
+    rename_like(A[i + 1].idx_set[0], mid);
+  }
+  center = (int)length - 1;
+}
+
+template <unsigned N>
+void qTensorTrain<double, N>::normalize() {   // qtt.cpp:899-909
+  if (center == -1) fail("normalize without a canonical centre is not supported");
+  double nm = A[center].norm();
+  A[center] *= 1.0 / nm;
+}
+
+template <unsigned N>
+double qTensorTrain<double, N>::position(int site) {   // qtt.cpp:932-994
+  if (center == -1) rc();
+  qtensor<double> U, V, S;
+  bool moved = false;
+  while (center != site) {
+    std::vector<qtensor_index> left, right;
+    qtensor_index mid;
+    moved = true;
+    if (center > site) {
+      std::string LN = link_name(_id, center);
+      for (auto& ix : A[center].idx_set) {
+        if (ix.name() == LN) { left.push_back(ix); mid = ix; } else right.push_back(ix);
+      }
+      svd(A[center], left, right, U, V, S, MoveFromRight);
+      bond_dims[center] = (unsigned)S.total_size();
+      A[center] = V;
+      rename_like(A[center].idx_set[0], mid);
+      A[center - 1] = A[center - 1] * U;
+      rename_like(A[center - 1].idx_set.back(), mid);
+      --center;
+    } else {
+      std::string LN = link_name(_id, center + 1);
+      for (auto& ix : A[center].idx_set) {
+        if (ix.name() == LN) { right.push_back(ix); mid = ix; } else left.push_back(ix);
+      }
+      svd(A[center], left, right, U, V, S, MoveFromLeft);
+      bond_dims[center + 1] = (unsigned)S.total_size();
+      A[center] = U;
+      rename_like(A[center].idx_set.back(), mid);
+      A[center + 1] = V * A[center + 1];
+      rename_like(A[center + 1].idx_set[0], mid);
+      ++center;
+    }
+  }
+  if (!moved) return 0.;
+  return calcEntropy(S);
+}
+
+template class qTensorTrain<double, 1>;
+template class qTensorTrain<double, 2>;
+
+// ---------------------------------------------------------------- DMRG
+// dmrg.cpp:56-84
+void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+              std::vector<qtensor<double>>& TL) {
+  const unsigned L = psi.length;
+  {
+    qtensor<double> le({psi.A[0].idx_set[0], H.A[0].idx_set[0], psi.A[0].idx_set[0]});
+    le.idx_set[0].primeLink(); le.idx_set[0].dag();
+    le.initBlock(); le.setOne(); le.dag();
+    TL[0] = std::move(le);
+  }
+  {
+    qtensor<double> re({psi.A[L - 1].idx_set.back(), H.A[L - 1].idx_set.back(), psi.A[L - 1].idx_set.back()});
+    re.idx_set[0].primeLink(); re.idx_set[0].dag();
+    re.initBlock(); re.setOne(); re.dag();
+    TR[L - 1] = std::move(re);
+  }
+  for (unsigned i = L - 1; i > 0; i--) {
+    qtensor<double> t1 = psi.A[i];
+    t1.conj(); t1.dag(); t1.prime();
+    qtensor<double> t4 = t1 * TR[i];
+    qtensor<double> t5 = t4 * H.A[i];
+    TR[i - 1] = t5 * psi.A[i];
+  }
+}
+
+// dmrg.cpp:491-509
+void updateEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+               std::vector<qtensor<double>>& TL, const unsigned& site, int& direction) {
+  qtensor<double> t1 = psi.A[site];
+  t1.conj(); t1.dag(); t1.prime();
+  if (direction == MoveFromLeft) {
+    qtensor<double> t4 = t1 * TL[site];
+    qtensor<double> t5 = t4 * H.A[site];
+    TL[site + 1] = t5 * psi.A[site];
+  } else {
+    qtensor<double> t4 = t1 * TR[site];
+    qtensor<double> t5 = t4 * H.A[site];
+    TR[site - 1] = t5 * psi.A[site];
+  }
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// dmrg.cpp:300-372
+void updateSite(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+                std::vector<qtensor<double>>& TL, const unsigned& site, double& energy,
+                int& direction, int max_bd, double cutoff, char mode, int search_space_size,
+                int max_restart, double tol, SiteLog* log) {
+  const unsigned a = (direction == MoveFromLeft) ? site : site - 1;
+  double t0 = now_s();
+  big_qtensor<double> A;
+  A.setLeft(&TL[a]);
+  A.addMid(&H.A[a]);
+  A.addMid(&H.A[a + 1]);
+  A.setRight(&TR[a + 1]);
+  qtensor<double> x = psi.A[a] * psi.A[a + 1];
+  DavidsonStats st;
+  energy = tensor_davidsonIT(A, x, search_space_size, max_restart, tol, mode, &st);
+  ttg_sync();
+  double t1 = now_s();
+  qtensor<double> S;
+  qtensor_index mid;
+  std::string LN = link_name(psi._id, a + 1);
+  for (auto& ix : psi.A[a].idx_set) if (ix.name() == LN) { mid = ix; break; }
+  unsigned kept = 0;
+  double tr = svd_bond(x, psi.A[a], psi.A[a + 1], mid, S, direction, cutoff, (long unsigned)max_bd, &kept);
+  psi.bond_dims[a + 1] = (unsigned)S.total_size();
+  psi.center = (direction == MoveFromLeft) ? (int)site + 1 : (int)site - 1;
+  ttg_sync();
+  double t2 = now_s();
+  if (log) { log->energy = energy; log->trunc_err = tr; log->kept = kept; log->n_matvec = st.n_matvec;
+             log->t_dav = t1 - t0; log->t_svd = t2 - t1; }
+}
+
+// dmrg.cpp:760-813
+double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
+            const std::vector<double>& cutoff, const std::vector<int>& max_restart, double tol,
+            DmrgLog* log) {
+  const int L = (int)H.length;
+  int direction;
+  const int search_space_size = 3;
+  psi.position(0);
+  psi.normalize();
+  std::vector<qtensor<double>> TR(L), TL(L);
+  buildEnv(psi, H, TR, TL);
+  double Energy = 0;
+  {
+    big_qtensor<double> A;
+    A.setLeft(&TL[0]);
+    A.addMid(&H.A[0]);
+    A.setRight(&TR[0]);
+    Energy = A.expec(psi.A[0]);
+    if (log) log->initial_energy = Energy;
+  }
+  for (int sweep = 0; sweep < num_sweeps; sweep++) {
+    const int lbd = sweep % (int)max_bd.size();
+    const int lc = sweep % (int)cutoff.size();
+    const int lmr = sweep % (int)max_restart.size();
+    for (int l = 2 * sweep; l < 2 * (sweep + 1); l++) {
+      double t0 = now_s();
+      direction = (l % 2 == 0) ? MoveFromLeft : MoveFromRight;
+      for (int i = 0; i < L - 1; i++) {
+        unsigned site = (direction == MoveFromLeft) ? (unsigned)i : (unsigned)(L - 1 - i);
+        SiteLog sl{};
+        updateSite(psi, H, TR, TL, site, Energy, direction, max_bd[lbd], cutoff[lc], 'S',
+                   search_space_size, max_restart[lmr], tol, &sl);
+        double te = now_s();
+        updateEnv(psi, H, TR, TL, site, direction);
+        ttg_sync();
+        if (log) {
+          log->bond_energy.push_back(sl.energy);
+          log->trunc_err.push_back(sl.trunc_err);
+          log->t_dav += sl.t_dav;
+          log->t_svd += sl.t_svd;
+          log->t_env += now_s() - te;
+          log->n_matvec += sl.n_matvec;
+        }
+      }
+      ttg_sync();
+      if (log) {
+        log->halfsweep_energy.push_back(Energy);
+        log->halfsweep_seconds.push_back(now_s() - t0);
+      }
+    }
+    psi.normalize();
+  }
+  psi.position(0);
+  return Energy;
+}
+
+}  // namespace ttgpu

@@ -10,8 +10,7 @@
 namespace ttgpu {
 
 void fail(const char* what) {
-  fprintf(stderr, "ttgpu: %s failed: %s\n", what, ttg_last_error());
-  abort();
+  throw Error(std::string(what) + " failed: " + ttg_last_error());
 }
 
 std::string to_string(const QN_t& q) {
@@ -190,6 +189,11 @@ qtensor<double> qtensor<double>::operator*(const qtensor& B) const {
   res.dev = adopt(c);
   res._initted = true;
   return res;
+}
+
+void qtensor<double>::dag() {
+  for (auto& i : idx_set) i.dag();
+  if (dev) check(ttg_tensor_dag(dev.get()), "dag");
 }
 
 qtensor<double>& qtensor<double>::operator*=(double c) {

@@ -14,6 +14,7 @@
 #pragma once
 #include <cstdint>
 #include <map>
+#include <stdexcept>
 #include <memory>
 #include <string>
 #include <utility>
@@ -23,6 +24,12 @@
 
 namespace ttgpu {
 
+// failure of a device call: throws ttgpu::Error with the C-ABI message (an
+// uncaught Error terminates, the reference's assert-and-abort behaviour;
+// the C entry points in c_api.cpp turn it into an error return instead)
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 [[noreturn]] void fail(const char* what);
 inline void check(int rc, const char* what) {
   if (rc != 0) fail(what);
@@ -128,7 +135,7 @@ class qtensor<double> {
   double normalize();                        // qtensor.cpp:1638-1647
   void prime(int inc = 1) { for (auto& i : idx_set) i.prime(inc); }
   void primeLink(int inc = 1) { for (auto& i : idx_set) i.primeLink(inc); }
-  void dag() { for (auto& i : idx_set) i.dag(); }
+  void dag();                                 // qtensor.cpp:1575-1581 (host + device arrows)
   void conj() {}
   size_t total_size() const;
   int nblocks() const;

@@ -74,6 +74,9 @@ def load(path: str = LIB_PATH):
         "ttg_heff_info": ([_p, C.POINTER(_d), _lp, _dp], _i),
         "ttg_heff_time": ([_p, _p, _p, _i, C.POINTER(_d)], _i),
         "ttg_heff_free": ([_p], _i),
+        "ttg_heff_stage_time": ([_p, _p, _i, _dp], _i),
+        "ttg_heff_tiles": ([_p, _ip, _ip], _i),
+        "ttg_same_blocks": ([_p, _p, C.POINTER(_i)], _i),
         "ttg_svd": ([_p, _i, _i, _d, _i64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                      C.POINTER(_d), C.POINTER(_i64)], _i),
         "ttg_tensor_index_download": ([_p, _i, C.POINTER(_i), _ip, _ip], _i),
@@ -284,6 +287,17 @@ class Heff:
         ms = _d(0)
         check(lib().ttg_heff_time(self.h, x.h, y.h, n, C.byref(ms)))
         return ms.value
+
+    def stage_time(self, x: DTensor, n: int):
+        out = np.zeros(3)
+        check(lib().ttg_heff_stage_time(self.h, x.h, n, out))
+        return out
+
+    def tiles(self):
+        cfg = np.zeros(3, np.int32)
+        nt = np.zeros(3, np.int32)
+        check(lib().ttg_heff_tiles(self.h, cfg, nt))
+        return cfg, nt
 
 
 def embed(src: DTensor, dst: DTensor):

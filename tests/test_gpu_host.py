@@ -1,0 +1,196 @@
+"""GPU parity of the C++ host layer (reference API over device blocks)
+against the oracle: tag-based contraction, MPO/MPS builders (bit-exact),
+sector SVD with truncation, Davidson, and two-site DMRG energies (vs the
+oracle and vs exact diagonalisation)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from scenarios import heff_inputs, mpo_bond
+from ttgpu import host as P
+from ttgpu.flat import INWARD, LINK, OUTWARD, SITE, make_index, random_tensor
+
+pytestmark = pytest.mark.gpu
+ED = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ed_energies.json")))
+
+
+def sec(qs, ds):
+    return [((q, 0), d) for q, d in zip(qs, ds)]
+
+
+def same_flat(a, b, exact=True, tol=1e-12):
+    assert [ix.name for ix in a.idx] == [ix.name for ix in b.idx]
+    assert [ix.arrow for ix in a.idx] == [ix.arrow for ix in b.idx]
+    assert [ix.level for ix in a.idx] == [ix.level for ix in b.idx]
+    assert [(ix.qn, ix.qdim) for ix in a.idx] == [(ix.qn, ix.qdim) for ix in b.idx]
+    assert np.array_equal(a.block_qi, b.block_qi)
+    if exact:
+        assert np.array_equal(a.data, b.data)
+    else:
+        assert np.linalg.norm(a.data - b.data) <= tol * max(np.linalg.norm(b.data), 1e-300)
+
+
+def test_contract_by_tags_matches_oracle():
+    rng = np.random.default_rng(2)
+    a = make_index(INWARD, "a", LINK, 0, sec([-1, 0, 1, 2], [3, 1, 4, 2]))
+    b = make_index(OUTWARD, "b", LINK, 0, sec([-2, 0, 2], [2, 3, 2]))
+    c = make_index(INWARD, "c", LINK, 0, sec([-1, 1, 3], [5, 2, 1]))
+    d = make_index(OUTWARD, "d", LINK, 2, sec([0, 1, 2], [2, 2, 3]))
+    e = make_index(INWARD, "e", LINK, 0, sec([-2, -1, 0, 1], [1, 2, 3, 2]))
+    A = random_tensor([a, b, c, d], rng, keep=0.7)
+    B = random_tensor([e, d.dag(), b.dag()], rng, keep=0.7)
+    got = (P.PTensor.from_flat(A) * P.PTensor.from_flat(B)).to_flat()
+    ref = (O.OTensor.from_flat(A) * O.OTensor.from_flat(B)).to_flat()
+    same_flat(got, ref, exact=False)
+
+
+@pytest.mark.parametrize("kind,Nx,Ny,yper,J2", [(1, 12, 1, False, 0.5), (0, 4, 4, True, 0.0)])
+def test_autompo_builder_bit_exact(kind, Nx, Ny, yper, J2):
+    N = Nx * Ny
+    pt = P.heisenberg_terms(kind, Nx, Ny, yper, 1.0, J2)
+    ot = O.heisenberg_terms(kind, Nx, Ny, yper, 1.0, J2)
+    for x, y in zip(pt, ot):
+        assert np.array_equal(x, y)
+    Hp, Ho = P.mpo_autompo(N, pt), O.mpo_autompo(N, ot)
+    for i in range(N):
+        same_flat(Hp.tensor(i).to_flat(), Ho.tensor(i).to_flat(), exact=True)
+
+
+def test_product_state_and_nn_mpo_bit_exact():
+    st = [i % 2 for i in range(8)]
+    pm, om = P.mps_product(st), O.mps_product(st)
+    for i in range(8):
+        same_flat(pm.tensor(i).to_flat(), om.tensor(i).to_flat(), exact=True)
+    hp, ho = P.mpo_heisenberg_nn(8), O.mpo_heisenberg_nn(8)
+    for i in range(8):
+        same_flat(hp.tensor(i).to_flat(), ho.tensor(i).to_flat(), exact=True)
+
+
+def two_site(seed, m=30):
+    # x = psi_a * psi_b with a random left-orthogonal-free psi pair
+    rng = np.random.default_rng(seed)
+    lsec = sec([-4, -2, 0, 2, 4], [2, 4, 6, 4, 2])
+    msec = sec([-3, -1, 1, 3], [3, 6, 6, 3])
+    rsec = sec([-4, -2, 0, 2, 4], [1, 3, 5, 3, 1])
+    l = make_index(INWARD, "ID1Link3", LINK, 0, lsec)
+    s1 = make_index(INWARD, "Site3", SITE, 0, sec([-1, 1], [1, 1]))
+    mo = make_index(OUTWARD, "ID1Link4", LINK, 0, msec)
+    s2 = make_index(INWARD, "Site4", SITE, 0, sec([-1, 1], [1, 1]))
+    r = make_index(OUTWARD, "ID1Link5", LINK, 0, rsec)
+    A = random_tensor([l, s1, mo], rng)
+    B = random_tensor([mo.dag(), s2, r], rng)
+    return A, B
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("cutoff,K", [(0.0, 0), (1e-3, 0), (1e-10, 7)])
+def test_svd_bond_matches_oracle(direction, cutoff, K):
+    A, B = two_site(11)
+    oA, oB = O.OTensor.from_flat(A), O.OTensor.from_flat(B)
+    pA, pB = P.PTensor.from_flat(A), P.PTensor.from_flat(B)
+    ox, px = oA * oB, pA * pB
+    otr, okept, oS = O.svd_bond(ox, oA, oB, "ID1Link4", direction, cutoff, K)
+    ptr, pkept, pS = P.svd_bond(px, pA, pB, "ID1Link4", direction, cutoff, K)
+    assert pkept == okept
+    assert ptr == pytest.approx(otr, abs=1e-12)
+    so = np.sort(oS.to_flat().data)
+    sp_ = np.sort(pS.to_flat().data)
+    # singular values from the FP64 Gram eigensolver carry an absolute error
+    # of about sqrt(eps) * s_max (s^2 to eps * s_max^2): tiny values kept at
+    # cutoff 0 are resolved only to that level
+    assert np.allclose(sp_, so, rtol=1e-10, atol=3e-8 * so.max())
+    # the truncated product U*V reproduces the oracle's (gauge-free check)
+    rec_p = (pA * pB).to_flat().to_dense()
+    rec_o = (oA * oB).to_flat().to_dense()
+    assert np.linalg.norm(rec_p - rec_o) <= 1e-10 * np.linalg.norm(rec_o)
+    # the unabsorbed factor is an isometry
+    iso = pA if direction == 0 else pB
+    f = iso.to_flat()
+    D = f.to_dense()
+    if direction == 0:
+        M = D.reshape(-1, D.shape[-1], order="F")
+        G = M.T @ M
+    else:
+        M = D.reshape(D.shape[0], -1, order="F")
+        G = M @ M.T
+    assert np.allclose(G, np.eye(G.shape[0]), atol=1e-12)
+
+
+def test_position_canonicalises_like_oracle():
+    st = [i % 2 for i in range(10)]
+    pm, om = P.mps_product(st), O.mps_product(st)
+    pm.position(0)
+    om.position(0)
+    assert pm.center() == om.center() == 0
+    assert np.array_equal(pm.bond_dims(), om.bond_dims())
+    assert pm.tensor(0).norm() == pytest.approx(om.tensor(0).norm(), rel=1e-13)
+
+
+def test_davidson_matches_oracle_on_a_dmrg_bond():
+    # environments of a real state: run 2 oracle sweeps, then compare one
+    # Davidson call on the same (L, W, W, R, x) through both implementations
+    N = 12
+    W1, W2 = mpo_bond("nn_chain", 5)
+    L, R, x = heff_inputs(W1, W2, 5, 24, 3, 1.5, seed=2)
+    # make L and R Hermitian environments: symmetrise over (l', l) per MPO leg pair
+    oL, oR, ox = (O.OTensor.from_flat(t) for t in (L, R, x))
+    big_o = O.OBig(oL, O.OTensor.from_flat(W1), O.OTensor.from_flat(W2), oR)
+    pL, pR, px = (P.PTensor.from_flat(t) for t in (L, R, x))
+    big_p = P.PBig(pL, P.PTensor.from_flat(W1), P.PTensor.from_flat(W2), pR)
+    assert big_p.size() == big_o.size()
+    eo, no = big_o.davidson(ox, 2, 1e-12)
+    ep, np_ = big_p.davidson(px, 2, 1e-12)
+    assert np_ == no
+    assert ep == pytest.approx(eo, rel=1e-10)
+    xo = ox.to_flat().to_dense()
+    xp = px.to_flat().to_dense()
+    assert np.linalg.norm(xo - xp) <= 1e-9
+
+
+def test_initial_energy_and_environments_match_oracle():
+    # psi.position(0) + buildEnv + the one-site expectation value of dmrg()
+    # (dmrg.cpp:766-782) through the host layer and through the oracle
+    N = 8
+    st = [i % 2 for i in range(N)]
+    r = P.dmrg(P.mps_product(st), P.mpo_heisenberg_nn(N), 0, [8], [1e-10], [2], 1e-10)
+    Eo, _, e0, _ = O.dmrg(O.mps_product(st), O.mpo_heisenberg_nn(N), 0, [8], [1e-10], [2], 1e-10)
+    assert r["E0"] == pytest.approx(e0, abs=1e-12)
+
+
+def test_dag_updates_device_arrows():
+    rng = np.random.default_rng(8)
+    a = make_index(INWARD, "a", LINK, 0, sec([-1, 1], [2, 3]))
+    b = make_index(OUTWARD, "b", LINK, 0, sec([-1, 1], [3, 2]))
+    c = make_index(INWARD, "c", LINK, 0, sec([-1, 1], [1, 4]))
+    A = random_tensor([a, b], rng)
+    B = random_tensor([b, c], rng)           # (b Out, c In) pairs with dag(A)'s b In
+    pt, ot = P.PTensor.from_flat(A), O.OTensor.from_flat(A)
+    pt.dag()
+    ot.dag()
+    got = (pt * P.PTensor.from_flat(B)).to_flat()
+    ref = (ot * O.OTensor.from_flat(B)).to_flat()
+    same_flat(got, ref, exact=False)
+    # full contraction <A, A> via A^dag' * A'
+    pt.prime()
+    u = P.PTensor.from_flat(A)
+    u.prime()
+    full = (pt * u).to_flat()
+    assert full.data.size == 1
+    assert full.data[0] == pytest.approx(np.dot(A.data, A.data), rel=1e-13)
+
+
+def test_dmrg_n20_matches_ed_and_oracle():
+    N = 20
+    st = [i % 2 for i in range(N)]
+    H = P.mpo_heisenberg_nn(N)
+    psi = P.mps_product(st)
+    r = P.dmrg(psi, H, 10, [16, 32, 64, 64], [1e-10], [2], 1e-10)
+    assert abs(r["E"] - ED["heisenberg_obc_N20"]) / N < 1e-9
+    assert r["E0"] == pytest.approx(-(N - 1) / 4.0)
+    Eo, hs, _, _ = O.dmrg(O.mps_product(st), O.mpo_heisenberg_nn(N), 10, [16, 32, 64, 64], [1e-10],
+                          [2], 1e-10)
+    assert abs(r["E"] - Eo) / N < 1e-10
+    assert np.allclose(r["halfsweep_E"][-4:], hs[-4:], atol=1e-9)
