@@ -159,6 +159,33 @@ void* tth_big_heff(void* b, void* x) {
     return bq->op();
   )
 }
+// device time per product (ms) over n graph replays, x widened to the
+// operator's block list first; ms3: the three stage kernels timed apart
+int tth_big_time(void* b, void* x, int n, double* ms, double* ms3) {
+  TTH_GUARD(-1,
+    auto* bq = (big_qtensor<double>*)b;
+    Q w = bq->widen(*(Q*)x);
+    Q y;
+    bq->product_into(w, y);
+    check(ttg_heff_time(bq->op(), w.dev.get(), y.dev.get(), n, ms), "heff_time");
+    if (ms3) check(ttg_heff_stage_time(bq->op(), w.dev.get(), n, ms3), "heff_stage_time");
+    return 0;
+  )
+}
+// flops per product, per-stage flops / task counts / tile shape / tile count
+int tth_big_info(void* b, void* x, double* flops, double* sflops3, long* ntasks3, int* cfg3,
+                 int* ntiles3) {
+  TTH_GUARD(-1,
+    auto* bq = (big_qtensor<double>*)b;
+    Q y;
+    bq->product_into(*(Q*)x, y);
+    int64_t nt[3];
+    check(ttg_heff_info(bq->op(), flops, nt, sflops3), "heff_info");
+    for (int k = 0; k < 3; k++) ntasks3[k] = (long)nt[k];
+    check(ttg_heff_tiles(bq->op(), cfg3, ntiles3), "heff_tiles");
+    return 0;
+  )
+}
 double tth_davidson(void* b, void* x, int max_restart, double tol, int* nmv, double* resid) {
   TTH_GUARD(kNaN,
     DavidsonStats st;

@@ -36,6 +36,8 @@ def _lib():
             "tth_big_new": ([_p, _p, _p, _p], _p), "tth_big_free": ([_p], None),
             "tth_big_product": ([_p, _p], _p), "tth_big_expec": ([_p, _p], _d),
             "tth_big_size": ([_p], _l), "tth_big_heff": ([_p, _p], _p),
+            "tth_big_time": ([_p, _p, _i, C.POINTER(_d), _dp], _i),
+            "tth_big_info": ([_p, _p, C.POINTER(_d), _dp, _lp, _ip, _ip], _i),
             "tth_davidson": ([_p, _p, _i, _d, C.POINTER(_i), C.POINTER(_d)], _d),
             "tth_svd_bond": ([_p, _p, _p, C.c_char_p, _i, _d, _l, C.POINTER(_i), C.POINTER(_p)], _d),
             "tth_mpo_heisenberg_nn": ([_i, _i], _p),
@@ -176,6 +178,22 @@ class PBig:
     def heff(self, x: PTensor):
         """The device operator (ttg_heff) planned for x's indices."""
         return _check(_lib().tth_big_heff(self.h, x.h), "heff")
+
+    def time(self, x: PTensor, n: int):
+        """(ms per product over n HIP-graph replays, ms of each stage kernel)"""
+        ms = _d(0)
+        ms3 = np.zeros(3)
+        _ok(_lib().tth_big_time(self.h, x.h, n, C.byref(ms), ms3), "time")
+        return ms.value, ms3
+
+    def info(self, x: PTensor):
+        fl = _d(0)
+        sf = np.zeros(3)
+        nt = np.zeros(3, np.int64)
+        cfg = np.zeros(3, np.int32)
+        ntl = np.zeros(3, np.int32)
+        _ok(_lib().tth_big_info(self.h, x.h, C.byref(fl), sf, nt, cfg, ntl), "info")
+        return dict(flops=fl.value, stage_flops=sf, ntasks=nt, tile_cfg=cfg, ntiles=ntl)
 
     def davidson(self, x: PTensor, max_restart=2, tol=1e-12):
         n = _i(0)

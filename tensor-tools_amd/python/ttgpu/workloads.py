@@ -1,0 +1,102 @@
+"""Seeded synthetic effective-Hamiltonian workloads (the matvec benchmarks).
+
+Sector model: a link of bond dimension m is split over Sz sectors s in
+[-S, S] (QN q0 = 2*Sz, q1 = 0) with dims proportional to
+exp(-s^2 / (2 sigma^2)), rounded, minimum 1; the rounding residue is spread
+one unit at a time over the largest sectors (ties to the lower Sz) until the
+dims sum to m exactly.
+
+L, R and the two-site vector x are filled block by block (block order = the
+reference's initBlock order) with N(0,1) FP64 values from numpy's PCG64
+seeded with seed+0 (L), seed+1 (R), seed+2 (x).  W holds the exact MPO
+coefficients from the AutoMPO builder of the host layer.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .flat import INWARD, LINK, OUTWARD, SITE, FlatTensor, legal_blocks, make_index
+
+# name: (lattice, Nx, Ny, y-periodic, J2, bond site, m, S, sigma)
+WORKLOADS = {
+    "j1j2_chain_N100_m512": ("j1j2", 100, 1, False, 0.5, 49, 512, 20, 6.0),
+    "heisenberg_cyl6x12_m2048": ("nn", 12, 6, True, 0.0, 35, 2048, 30, 9.0),
+    "j1j2_cyl8x16_m8192": ("j1j2", 16, 8, True, 0.5, 63, 8192, 40, 12.0),
+}
+
+
+def sector_dims(m: int, smax: int, sigma: float):
+    s = np.arange(-smax, smax + 1)
+    w = np.exp(-s.astype(float) ** 2 / (2 * sigma * sigma))
+    d = np.maximum(1, np.round(m * w / w.sum())).astype(int)
+    order = sorted(range(len(s)), key=lambda i: (-d[i], s[i]))
+    k = 0
+    while d.sum() != m:
+        i = order[k % len(order)]
+        if d.sum() < m:
+            d[i] += 1
+        elif d[i] > 1:
+            d[i] -= 1
+        k += 1
+    return [(2 * int(q), int(dd)) for q, dd in zip(s, d)]
+
+
+def link_index(arrow, name, level, sectors):
+    return make_index(arrow, name, LINK, level, [((q, 0), d) for q, d in sectors])
+
+
+def site_index(arrow, name, level):
+    return make_index(arrow, name, SITE, level, [((-1, 0), 1), ((1, 0), 1)])
+
+
+def fill(idx, seed):
+    rng = np.random.default_rng(seed)
+    bq = legal_blocks(idx)
+    t = FlatTensor([ix.copy() for ix in idx], bq, np.zeros(0))
+    t.data = rng.standard_normal(int(t.block_sizes().sum()))
+    return t
+
+
+def heff_inputs(W1: FlatTensor, W2: FlatTensor, site: int, m: int, smax: int, sigma: float,
+                seed: int, psi_id: int = 1):
+    """L, R, x for the bond (site, site+1) around MPO tensors W1 = W_site and
+    W2 = W_site+1 (index order: link In, site Out lvl0, site In lvl1, link
+    Out).  Index names follow the reference: ID<id>Link<i>, Site<i>; the
+    environments have the reference's layouts TL = (l' In lvl1, w Out, l Out)
+    and TR = (r' Out lvl1, w In, r In) (dmrg.cpp:60-74)."""
+    secs = sector_dims(m, smax, sigma)
+    l = link_index(INWARD, f"ID{psi_id}Link{site}", 0, secs)
+    r = link_index(OUTWARD, f"ID{psi_id}Link{site + 2}", 0, secs)
+    si = site_index(INWARD, f"Site{site}", 0)
+    sj = site_index(INWARD, f"Site{site + 1}", 0)
+    wl = W1.idx[0].dag()
+    wr = W2.idx[3].dag()
+    lp = l.copy()
+    lp.level = 1
+    rp = r.copy()
+    rp.level = 1
+    L = fill([lp, wl, l.dag()], seed)
+    R = fill([rp, wr, r.dag()], seed + 1)
+    x = fill([l, si, sj, r], seed + 2)
+    return L, R, x
+
+
+def lattice_terms(builder, lattice: str, Nx: int, Ny: int, yper: bool, J2: float):
+    kind = 1 if lattice == "j1j2" else 0
+    return builder.heisenberg_terms(kind, Nx, Ny, yper, 1.0, J2)
+
+
+def mpo_bond(builder, name: str):
+    """(W_site, W_site+1) as flat tensors from `builder` (the host layer's
+    Python binding, or the oracle's in tests)."""
+    lat, Nx, Ny, yper, J2, site, *_ = WORKLOADS[name]
+    terms = lattice_terms(builder, lat, Nx, Ny, yper, J2)
+    H = builder.mpo_autompo(Nx * Ny, terms)
+    return H.tensor(site).to_flat(), H.tensor(site + 1).to_flat()
+
+
+def workload_inputs(builder, name: str, seed: int = 7):
+    lat, Nx, Ny, yper, J2, site, m, smax, sigma = WORKLOADS[name]
+    W1, W2 = mpo_bond(builder, name)
+    L, R, x = heff_inputs(W1, W2, site, m, smax, sigma, seed)
+    return dict(W1=W1, W2=W2, L=L, R=R, x=x, site=site, m=m)
