@@ -1,17 +1,26 @@
 // gemm.hip -- persistent grouped block GEMM over a plan's tile list.
 //
-// Every uneven block pair of a contraction runs in ONE launch.  A tile is a
-// TM x TN piece of one output block; the workgroup owning it walks all of
-// the block's tasks (the K contributions, in the reference's mid order) and
-// writes the tile once, so no atomics and no zero-fill pass are needed
-// (blocks without tasks are written as zeros).  Operand tiles are staged
+// Every uneven block pair of a contraction runs in ONE launch.  A work item
+// is a TM x TN piece of one output block and a range of its k walk (all of
+// the block's tasks, the K contributions in the reference's mid order).  An
+// item owning the whole walk writes the tile once, so no atomics and no
+// zero-fill pass are needed (blocks without tasks are written as zeros);
+// long walks are split into pieces that write partial tiles, added in fixed
+// order by k_split_reduce (deterministic split-K).  Operand tiles are staged
 // through LDS; the next k-slice is prefetched into registers while the MFMA
 // pipe works on the current one.
+//
+// K walk: a task's contraction index is k = k0 + e0*k1 (two stride
+// components).  k-tiles never straddle a k1 boundary: for each k1 the
+// k0 range is cut into chunks of TK (the last one partial, zero-filled), so
+// an operand element's address is base + row_offset + (c + kl)*s0 + k1*s1
+// with no division inside the loop; row offsets (the M or N mixed-radix
+// decomposition) are computed once per task.
 //
 // MFMA building block: 16x16 output fragment per wave-instruction.
 //   FP64:   v_mfma_f64_16x16x4_f64      (A,B one double per lane)
 //   FP32:   v_mfma_f32_16x16x4_f32      (A,B one float per lane)
-//   BF16X3: v_mfma_f32_16x16x32_bf16 x3 (hi*hi + hi*lo + lo*hi, FP32 acc)
+//   BF16X3: v_mfma_f32_16x16x32_bf16 x3 (lo*hi + hi*lo + hi*hi, FP32 acc)
 // Operand lane layout (k-slice of 4):  lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; result register r of lane l holds D[row][l%16] with
 // row = l/16 + 4r for f64 and 4*(l/16) + r for the f32-accumulating ops.
@@ -28,9 +37,6 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 template <typename T>
 struct MfmaOp;
-// row of result register r in lane l of a 16x16 fragment (column = l%16):
-// the f64 MFMA interleaves rows across registers, the f32 ones keep each
-// lane's four rows contiguous (both checked by an in-bounds layout probe).
 template <>
 struct MfmaOp<double> {
   typedef v4d acc_t;
@@ -54,25 +60,76 @@ __device__ __forceinline__ int64_t off2(int x, int e0, int64_t s0, int64_t s1) {
 
 constexpr int kThreads = 256;
 
+// k-walk state of one tile: task t, second k component k1, chunk start c
+struct KWalk {
+  int t, k1, c;
+};
+__device__ __forceinline__ void kwalk_next(KWalk& w, const GemmTask& g, int TK) {
+  w.c += TK;
+  if (w.c >= g.ek0) {
+    w.c = 0;
+    if (++w.k1 >= g.K / g.ek0) {
+      w.k1 = 0;
+      w.t++;
+    }
+  }
+}
+// position of k-tile kt of block cb's walk: the last task whose first k-tile
+// is <= kt (tasks[t0..t1) hold ascending kt0), then (k1, chunk) inside it
+__device__ __forceinline__ KWalk kwalk_start(const GemmTask* __restrict__ tasks, const CBlock& cb,
+                                             int kt, int TK) {
+  int lo = cb.t0, hi = cb.t1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tasks[mid].kt0 <= kt) lo = mid;
+    else hi = mid;
+  }
+  const int ek0 = tasks[lo].ek0, local = kt - tasks[lo].kt0;
+  const int nch = (ek0 + TK - 1) / TK;
+  return KWalk{lo, local / nch, (local % nch) * TK};
+}
+
+// Work items are taken with stride gridDim.x starting from a permuted block
+// id: workgroups are dispatched round-robin over the 8 XCDs, so block b runs
+// on XCD b % 8; the permutation gives each XCD a run of consecutive items
+// (neighbouring tiles of one output block share their A rows / B columns).
+__device__ __forceinline__ int xcd_first_item() {
+  const int G = gridDim.x, b = blockIdx.x;
+  return (G & 7) ? b : (b & 7) * (G >> 3) + (b >> 3);
+}
+
 // ------------------------------------------------------------------ FP64/FP32
-template <typename T, int TM, int TN, bool AM, bool BK, typename OutT>
+// WM x (4/WM) waves; each wave owns (TM/WM) x (TN/WN) of the tile.
+template <typename T, int TM, int TN, int WM, bool AM, bool BK, typename OutT>
 __global__ void __launch_bounds__(kThreads)
 k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cblk,
        const GemmTask* __restrict__ tasks, const T* __restrict__ A, const T* __restrict__ B,
-       OutT* __restrict__ C) {
+       OutT* __restrict__ C, T* __restrict__ part) {
   constexpr int TK = 16;
-  constexpr int FM = TM / 32, FN = TN / 32;   // 16x16 fragments per wave (2x2 waves)
+  constexpr int WN = 4 / WM;
+  constexpr int FM = TM / (16 * WM), FN = TN / (16 * WN);
   constexpr int NA = TM * TK / kThreads, NB = TN * TK / kThreads;
   constexpr int LDA = TM + 1, LDB = TN + 1;
+  static_assert(FM >= 1 && FN >= 1 && NA >= 1 && NB >= 1, "tile shape");
   typedef typename MfmaOp<T>::acc_t acc_t;
   __shared__ T As[TK * LDA];
   __shared__ T Bs[TK * LDB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WM, wn = wave / WM;
+  // fixed per-thread coordinates inside the tile
+  // A: AM -> one row, NA k-rows;  !AM -> one k-row, NA rows
+  const int a_r0 = AM ? tid % TM : tid / TK;
+  const int a_k0 = AM ? tid / TM : tid % TK;
+  constexpr int a_rstep = AM ? 0 : kThreads / TK;
+  constexpr int a_kstep = AM ? kThreads / TM : 0;
+  const int b_c0 = BK ? tid / TK : tid % TN;
+  const int b_k0 = BK ? tid % TK : tid / TN;
+  constexpr int b_cstep = BK ? kThreads / TK : 0;
+  constexpr int b_kstep = BK ? 0 : kThreads / TN;
 
-  for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+  for (int ti = xcd_first_item(); ti < ntiles; ti += gridDim.x) {
     const Tile tl = tiles[ti];
     const CBlock cb = cblk[tl.c];
     acc_t acc[FM][FN];
@@ -82,82 +139,79 @@ k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cb
       for (int j = 0; j < FN; j++) acc[i][j] = acc_t{0, 0, 0, 0};
 
     T ra[NA], rb[NB];
-    int t = cb.t0, kk = 0;
-    auto load = [&](int tt, int k0) {
-      const GemmTask g = tasks[tt];
+    int64_t aro[NA], bco[NB];       // row / column offsets of the current task (-1: outside)
+    int cur_task = -1;
+    GemmTask g;
+    auto prepare = [&](int tt) {    // per-task row / column offsets
+      g = tasks[tt];
+      cur_task = tt;
 #pragma unroll
       for (int i = 0; i < NA; i++) {
-        int m, k;
-        if (AM) { m = tid % TM; k = tid / TM + (kThreads / TM) * i; }
-        else { k = tid % TK; m = tid / TK + (kThreads / TK) * i; }
-        const int mg = tl.m0 + m, kg = k0 + k;
-        T v = T(0);
-        if (mg < cb.M && kg < g.K)
-          v = A[g.a_off + off2(mg, g.em0, g.sam0, g.sam1) + off2(kg, g.ek0, g.sak0, g.sak1)];
-        ra[i] = v;
+        const int mg = tl.m0 + a_r0 + a_rstep * i;
+        aro[i] = mg < cb.M ? g.a_off + off2(mg, g.em0, g.sam0, g.sam1) : -1;
       }
 #pragma unroll
       for (int i = 0; i < NB; i++) {
-        int n, k;
-        if (BK) { k = tid % TK; n = tid / TK + (kThreads / TK) * i; }
-        else { n = tid % TN; k = tid / TN + (kThreads / TN) * i; }
-        const int ng = tl.n0 + n, kg = k0 + k;
-        T v = T(0);
-        if (ng < cb.N && kg < g.K)
-          v = B[g.b_off + off2(kg, g.ek0, g.sbk0, g.sbk1) + off2(ng, g.en0, g.sbn0, g.sbn1)];
-        rb[i] = v;
+        const int ng = tl.n0 + b_c0 + b_cstep * i;
+        bco[i] = ng < cb.N ? g.b_off + off2(ng, g.en0, g.sbn0, g.sbn1) : -1;
       }
     };
-    bool have = t < cb.t1;
-    if (have) load(t, kk);
-    while (have) {
-      __syncthreads();
+    auto load = [&](const KWalk& w) {
+      if (w.t != cur_task) prepare(w.t);
+      const int64_t ka1 = (int64_t)w.k1 * g.sak1, kb1 = (int64_t)w.k1 * g.sbk1;
 #pragma unroll
       for (int i = 0; i < NA; i++) {
-        int m, k;
-        if (AM) { m = tid % TM; k = tid / TM + (kThreads / TM) * i; }
-        else { k = tid % TK; m = tid / TK + (kThreads / TK) * i; }
-        As[k * LDA + m] = ra[i];
+        const int kk = w.c + a_k0 + a_kstep * i;
+        ra[i] = (aro[i] >= 0 && kk < g.ek0) ? A[aro[i] + (int64_t)kk * g.sak0 + ka1] : T(0);
       }
 #pragma unroll
       for (int i = 0; i < NB; i++) {
-        int n, k;
-        if (BK) { k = tid % TK; n = tid / TK + (kThreads / TK) * i; }
-        else { n = tid % TN; k = tid / TN + (kThreads / TN) * i; }
-        Bs[k * LDB + n] = rb[i];
+        const int kk = w.c + b_k0 + b_kstep * i;
+        rb[i] = (bco[i] >= 0 && kk < g.ek0) ? B[bco[i] + (int64_t)kk * g.sbk0 + kb1] : T(0);
       }
+    };
+    int left = tl.kt1 - tl.kt0;   // k-tiles still to stage
+    KWalk w{cb.t0, 0, 0};
+    if (left > 0) {
+      w = kwalk_start(tasks, cb, tl.kt0, TK);
+      load(w);
+    }
+    while (left > 0) {
       __syncthreads();
-      kk += TK;
-      if (kk >= tasks[t].K) {
-        t++;
-        kk = 0;
-      }
-      have = t < cb.t1;
-      if (have) load(t, kk);
 #pragma unroll
-      for (int k4 = 0; k4 < TK; k4 += 4) {
+      for (int i = 0; i < NA; i++) As[(a_k0 + a_kstep * i) * LDA + a_r0 + a_rstep * i] = ra[i];
+#pragma unroll
+      for (int i = 0; i < NB; i++) Bs[(b_k0 + b_kstep * i) * LDB + b_c0 + b_cstep * i] = rb[i];
+      __syncthreads();
+      const int kvalid = min(TK, g.ek0 - w.c);   // uniform: skip all-zero k-steps
+      kwalk_next(w, g, TK);
+      if (--left > 0) load(w);
+      for (int k4 = 0; k4 < kvalid; k4 += 4) {
         const int kr = k4 + (lane >> 4);
         T af[FM], bf[FN];
 #pragma unroll
-        for (int i = 0; i < FM; i++) af[i] = As[kr * LDA + wm * (TM / 2) + i * 16 + (lane & 15)];
+        for (int i = 0; i < FM; i++) af[i] = As[kr * LDA + wm * (TM / WM) + i * 16 + (lane & 15)];
 #pragma unroll
-        for (int j = 0; j < FN; j++) bf[j] = Bs[kr * LDB + wn * (TN / 2) + j * 16 + (lane & 15)];
+        for (int j = 0; j < FN; j++) bf[j] = Bs[kr * LDB + wn * (TN / WN) + j * 16 + (lane & 15)];
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll
           for (int j = 0; j < FN; j++) acc[i][j] = MfmaOp<T>::mac(af[i], bf[j], acc[i][j]);
       }
     }
-    // epilogue: C block is column-major M x N
+    // epilogue: C block is column-major M x N; a split piece writes its
+    // whole TM x TN partial (column-major) to its slot instead
+    T* pt = tl.slot >= 0 ? part + (int64_t)tl.slot * (TM * TN) : nullptr;
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int n = tl.n0 + wn * (TN / 2) + j * 16 + (lane & 15);
+        const int nl = wn * (TN / WN) + j * 16 + (lane & 15), n = tl.n0 + nl;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int m = tl.m0 + wm * (TM / 2) + i * 16 + MfmaOp<T>::row(lane, r);
-          if (m < cb.M && n < cb.N) C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)acc[i][j][r];
+          const int ml = wm * (TM / WM) + i * 16 + MfmaOp<T>::row(lane, r), m = tl.m0 + ml;
+          if (pt) pt[ml + TM * nl] = acc[i][j][r];
+          else if (m < cb.M && n < cb.N) C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)acc[i][j][r];
         }
       }
   }
@@ -169,17 +223,21 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
-template <int TM, int TN, bool AM, bool BK, typename OutT>
+// Operands are FP32 in HBM, split into hi/lo bf16 planes when staged into
+// LDS (k-contiguous rows of 32, padded to 40 to spread banks).
+template <int TM, int TN, int WM, bool AM, bool BK, typename OutT>
 __global__ void __launch_bounds__(kThreads)
 k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cblk,
               const GemmTask* __restrict__ tasks, const float* __restrict__ A,
-              const float* __restrict__ B, OutT* __restrict__ C) {
+              const float* __restrict__ B, OutT* __restrict__ C, float* __restrict__ part) {
   constexpr int TK = 32;
-  constexpr int FM = TM / 32, FN = TN / 32;
-  constexpr int KP = TK + 8;                     // padded k-row of bf16 (80 B)
+  constexpr int WN = 4 / WM;
+  constexpr int FM = TM / (16 * WM), FN = TN / (16 * WN);
+  constexpr int KP = TK + 8;
   constexpr int NGA = TM * TK / 8, NGB = TN * TK / 8;   // 8-element k-groups per tile
   constexpr int GA = (NGA + kThreads - 1) / kThreads;
   constexpr int GB = (NGB + kThreads - 1) / kThreads;
+  static_assert(FM >= 1 && FN >= 1, "tile shape");
   __shared__ __attribute__((aligned(16))) __bf16 AsH[TM * KP];
   __shared__ __attribute__((aligned(16))) __bf16 AsL[TM * KP];
   __shared__ __attribute__((aligned(16))) __bf16 BsH[TN * KP];
@@ -187,9 +245,19 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WM, wn = wave / WM;
+  // thread -> (row, k-group) per load slot i: row-contiguous operands put
+  // lanes along the row; k-contiguous ones put 4 lanes on a row's 32 k
+  auto a_map = [&](int i, int& m, int& kg) {
+    const int e = tid + kThreads * i;
+    if (AM) { m = e % TM; kg = e / TM; } else { kg = e % 4; m = e / 4; }
+  };
+  auto b_map = [&](int i, int& n, int& kg) {
+    const int e = tid + kThreads * i;
+    if (BK) { kg = e % 4; n = e / 4; } else { n = e % TN; kg = e / TN; }
+  };
 
-  for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+  for (int ti = xcd_first_item(); ti < ntiles; ti += gridDim.x) {
     const Tile tl = tiles[ti];
     const CBlock cb = cblk[tl.c];
     v4f acc[FM][FN];
@@ -198,49 +266,58 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
 #pragma unroll
       for (int j = 0; j < FN; j++) acc[i][j] = v4f{0, 0, 0, 0};
     float ra[GA][8], rb[GB][8];
-    int t = cb.t0, kk = 0;
-    // thread -> (row, k-group): row-contiguous operands put lanes along the
-    // row; k-contiguous ones put 4 consecutive lanes on one row's 32 k.
-    auto a_map = [&](int i, int& m, int& kg) {
-      const int e = tid + kThreads * i;
-      if (AM) { m = e % TM; kg = e / TM; } else { kg = e % 4; m = e / 4; }
-    };
-    auto b_map = [&](int i, int& n, int& kg) {
-      const int e = tid + kThreads * i;
-      if (BK) { kg = e % 4; n = e / 4; } else { n = e % TN; kg = e / TN; }
-    };
-    auto load = [&](int tt, int k0) {
-      const GemmTask g = tasks[tt];
+    int64_t aro[GA], bco[GB];
+    int cur_task = -1;
+    GemmTask g;
+    auto prepare = [&](int tt) {
+      g = tasks[tt];
+      cur_task = tt;
 #pragma unroll
       for (int i = 0; i < GA; i++) {
         int m, kg;
         a_map(i, m, kg);
         const int mg = tl.m0 + m;
-        const bool mok = mg < cb.M && tid + kThreads * i < NGA;
-        const int64_t mo = mok ? off2(mg, g.em0, g.sam0, g.sam1) : 0;
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int kq = k0 + kg * 8 + u;
-          ra[i][u] = (mok && kq < g.K) ? A[g.a_off + mo + off2(kq, g.ek0, g.sak0, g.sak1)] : 0.f;
-        }
+        aro[i] = (tid + kThreads * i < NGA && mg < cb.M) ? g.a_off + off2(mg, g.em0, g.sam0, g.sam1) : -1;
       }
 #pragma unroll
       for (int i = 0; i < GB; i++) {
         int n, kg;
         b_map(i, n, kg);
         const int ng = tl.n0 + n;
-        const bool nok = ng < cb.N && tid + kThreads * i < NGB;
-        const int64_t no = nok ? off2(ng, g.en0, g.sbn0, g.sbn1) : 0;
+        bco[i] = (tid + kThreads * i < NGB && ng < cb.N) ? g.b_off + off2(ng, g.en0, g.sbn0, g.sbn1) : -1;
+      }
+    };
+    auto load = [&](const KWalk& w) {
+      if (w.t != cur_task) prepare(w.t);
+      const int64_t ka1 = (int64_t)w.k1 * g.sak1, kb1 = (int64_t)w.k1 * g.sbk1;
+#pragma unroll
+      for (int i = 0; i < GA; i++) {
+        int m, kg;
+        a_map(i, m, kg);
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int kq = k0 + kg * 8 + u;
-          rb[i][u] = (nok && kq < g.K) ? B[g.b_off + no + off2(kq, g.ek0, g.sbk0, g.sbk1)] : 0.f;
+          const int kk = w.c + kg * 8 + u;
+          ra[i][u] = (aro[i] >= 0 && kk < g.ek0) ? A[aro[i] + (int64_t)kk * g.sak0 + ka1] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < GB; i++) {
+        int n, kg;
+        b_map(i, n, kg);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int kk = w.c + kg * 8 + u;
+          rb[i][u] = (bco[i] >= 0 && kk < g.ek0) ? B[bco[i] + (int64_t)kk * g.sbk0 + kb1] : 0.f;
         }
       }
     };
-    bool have = t < cb.t1;
-    if (have) load(t, kk);
-    while (have) {
+    int left = tl.kt1 - tl.kt0;   // k-tiles still to stage
+    KWalk w{cb.t0, 0, 0};
+    if (left > 0) {
+      w = kwalk_start(tasks, cb, tl.kt0, TK);
+      load(w);
+    }
+    while (left > 0) {
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < GA; i++) {
@@ -275,24 +352,19 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
         *(v8bf*)&BsL[n * KP + kg * 8] = l;
       }
       __syncthreads();
-      kk += TK;
-      if (kk >= tasks[t].K) {
-        t++;
-        kk = 0;
-      }
-      have = t < cb.t1;
-      if (have) load(t, kk);
+      kwalk_next(w, g, TK);
+      if (--left > 0) load(w);
       const int kq = 8 * (lane >> 4);
       v8bf ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) {
-        const int row = wm * (TM / 2) + i * 16 + (lane & 15);
+        const int row = wm * (TM / WM) + i * 16 + (lane & 15);
         ah[i] = *(const v8bf*)&AsH[row * KP + kq];
         al[i] = *(const v8bf*)&AsL[row * KP + kq];
       }
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int col = wn * (TN / 2) + j * 16 + (lane & 15);
+        const int col = wn * (TN / WN) + j * 16 + (lane & 15);
         bh[j] = *(const v8bf*)&BsH[col * KP + kq];
         bl[j] = *(const v8bf*)&BsL[col * KP + kq];
       }
@@ -305,18 +377,46 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
         }
     }
+    float* pt = tl.slot >= 0 ? part + (int64_t)tl.slot * (TM * TN) : nullptr;
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int n = tl.n0 + wn * (TN / 2) + j * 16 + (lane & 15);
+        const int nl = wn * (TN / WN) + j * 16 + (lane & 15), n = tl.n0 + nl;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int m = tl.m0 + wm * (TM / 2) + i * 16 + 4 * (lane >> 4) + r;
-          if (m < cb.M && n < cb.N) C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)acc[i][j][r];
+          const int ml = wm * (TM / WM) + i * 16 + 4 * (lane >> 4) + r, m = tl.m0 + ml;
+          if (pt) pt[ml + TM * nl] = acc[i][j][r];
+          else if (m < cb.M && n < cb.N) C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)acc[i][j][r];
         }
       }
   }
+}
+
+// ------------------------------------------------------------------ split-K
+// C tile of a split group = sum of its partial slots, added in slot order
+template <typename AccT, typename OutT>
+__global__ void __launch_bounds__(256)
+k_split_reduce(const SplitGroup* __restrict__ groups, const CBlock* __restrict__ cblk,
+               const AccT* __restrict__ part, int TM, int TN, OutT* __restrict__ C) {
+  const SplitGroup gr = groups[blockIdx.x];
+  const CBlock cb = cblk[gr.c];
+  const int64_t tile = (int64_t)TM * TN;
+  for (int e = threadIdx.x; e < TM * TN; e += blockDim.x) {
+    const int m = gr.m0 + e % TM, n = gr.n0 + e / TM;
+    if (m >= cb.M || n >= cb.N) continue;
+    const AccT* src = part + (int64_t)gr.slot0 * tile + e;
+    AccT sum = src[0];
+    for (int k = 1; k < gr.ns; k++) sum += src[k * tile];
+    C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)sum;
+  }
+}
+
+template <typename AccT, typename OutT>
+static void launch_reduce(const ttg_plan_s& p, void* C, hipStream_t s) {
+  if (p.ngroups == 0) return;
+  hipLaunchKernelGGL((k_split_reduce<AccT, OutT>), dim3(p.ngroups), dim3(256), 0, s, p.d_groups,
+                     p.d_cblk, (const AccT*)p.d_part, cfg_tm(p.cfg), cfg_tn(p.cfg), (OutT*)C);
 }
 
 // ------------------------------------------------------------------ dispatch
@@ -325,12 +425,13 @@ int gemm_grid(const ttg_plan_s& p) {
   return std::max(1, std::min(g, p.ntiles));
 }
 
-template <typename T, int TM, int TN, typename OutT>
+template <typename T, int TM, int TN, int WM, typename OutT>
 static void launch_mode(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
   dim3 grid(gemm_grid(p)), block(kThreads);
-#define TTG_L(AMv, BKv)                                                                         \
-  hipLaunchKernelGGL((k_gemm<T, TM, TN, AMv, BKv, OutT>), grid, block, 0, s, p.d_tiles, p.ntiles, \
-                     p.d_cblk, p.d_tasks, (const T*)A, (const T*)B, (OutT*)C)
+#define TTG_L(AMv, BKv)                                                                           \
+  hipLaunchKernelGGL((k_gemm<T, TM, TN, WM, AMv, BKv, OutT>), grid, block, 0, s, p.d_tiles,       \
+                     p.ntiles, p.d_cblk, p.d_tasks, (const T*)A, (const T*)B, (OutT*)C,         \
+                     (T*)p.d_part)
   if (p.a_mcontig && p.b_kcontig) TTG_L(true, true);
   else if (p.a_mcontig) TTG_L(true, false);
   else if (p.b_kcontig) TTG_L(false, true);
@@ -338,57 +439,68 @@ static void launch_mode(const ttg_plan_s& p, const void* A, const void* B, void*
 #undef TTG_L
 }
 
-template <int TM, int TN, typename OutT>
+template <int TM, int TN, int WM, typename OutT>
 static void launch_bf16(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
   dim3 grid(gemm_grid(p)), block(kThreads);
-#define TTG_L(AMv, BKv)                                                                       \
-  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, AMv, BKv, OutT>), grid, block, 0, s, p.d_tiles,    \
-                     p.ntiles, p.d_cblk, p.d_tasks, (const float*)A, (const float*)B, (OutT*)C)
+#define TTG_L(AMv, BKv)                                                                          \
+  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>), grid, block, 0, s, p.d_tiles, \
+                     p.ntiles, p.d_cblk, p.d_tasks, (const float*)A, (const float*)B, (OutT*)C, \
+                     (float*)p.d_part)
   if (p.a_mcontig && p.b_kcontig) TTG_L(true, true);
   else if (p.a_mcontig) TTG_L(true, false);
   else if (p.b_kcontig) TTG_L(false, true);
   else TTG_L(false, false);
 #undef TTG_L
+}
+
+template <typename T, typename OutT>
+static void by_cfg(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
+  switch (p.cfg) {
+    case TILE_64x64: launch_mode<T, 64, 64, 2, OutT>(p, A, B, C, s); break;
+    case TILE_32x32: launch_mode<T, 32, 32, 2, OutT>(p, A, B, C, s); break;
+    case TILE_64x32: launch_mode<T, 64, 32, 2, OutT>(p, A, B, C, s); break;
+    case TILE_32x64: launch_mode<T, 32, 64, 2, OutT>(p, A, B, C, s); break;
+    case TILE_128x64: launch_mode<T, 128, 64, 2, OutT>(p, A, B, C, s); break;
+    case TILE_256x16: launch_mode<T, 256, 16, 4, OutT>(p, A, B, C, s); break;
+    case TILE_128x16: launch_mode<T, 128, 16, 4, OutT>(p, A, B, C, s); break;
+    default: launch_mode<T, 64, 16, 4, OutT>(p, A, B, C, s); break;
+  }
 }
 
 template <typename OutT>
-static int dispatch(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
-  switch (p.prec) {
-    case TTG_FP64:
-      switch (p.cfg) {
-        case TILE_64x64: launch_mode<double, 64, 64, OutT>(p, A, B, C, s); break;
-        case TILE_32x32: launch_mode<double, 32, 32, OutT>(p, A, B, C, s); break;
-        case TILE_64x32: launch_mode<double, 64, 32, OutT>(p, A, B, C, s); break;
-        case TILE_32x64: launch_mode<double, 32, 64, OutT>(p, A, B, C, s); break;
-        default: launch_mode<double, 128, 64, OutT>(p, A, B, C, s); break;
-      }
-      break;
-    case TTG_FP32:
-      switch (p.cfg) {
-        case TILE_64x64: launch_mode<float, 64, 64, OutT>(p, A, B, C, s); break;
-        case TILE_32x32: launch_mode<float, 32, 32, OutT>(p, A, B, C, s); break;
-        case TILE_64x32: launch_mode<float, 64, 32, OutT>(p, A, B, C, s); break;
-        case TILE_32x64: launch_mode<float, 32, 64, OutT>(p, A, B, C, s); break;
-        default: launch_mode<float, 128, 64, OutT>(p, A, B, C, s); break;
-      }
-      break;
-    default:
-      switch (p.cfg) {
-        case TILE_64x64: launch_bf16<64, 64, OutT>(p, A, B, C, s); break;
-        case TILE_32x32: launch_bf16<32, 32, OutT>(p, A, B, C, s); break;
-        case TILE_64x32: launch_bf16<64, 32, OutT>(p, A, B, C, s); break;
-        case TILE_32x64: launch_bf16<32, 64, OutT>(p, A, B, C, s); break;
-        default: launch_bf16<128, 64, OutT>(p, A, B, C, s); break;
-      }
+static void by_cfg_bf16(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
+  switch (p.cfg) {
+    case TILE_64x64: launch_bf16<64, 64, 2, OutT>(p, A, B, C, s); break;
+    case TILE_32x32: launch_bf16<32, 32, 2, OutT>(p, A, B, C, s); break;
+    case TILE_64x32: launch_bf16<64, 32, 2, OutT>(p, A, B, C, s); break;
+    case TILE_32x64: launch_bf16<32, 64, 2, OutT>(p, A, B, C, s); break;
+    case TILE_128x64: launch_bf16<128, 64, 2, OutT>(p, A, B, C, s); break;
+    case TILE_256x16: launch_bf16<256, 16, 4, OutT>(p, A, B, C, s); break;
+    case TILE_128x16: launch_bf16<128, 16, 4, OutT>(p, A, B, C, s); break;
+    default: launch_bf16<64, 16, 4, OutT>(p, A, B, C, s); break;
   }
-  return 0;
 }
 
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s) {
   if (p.ntiles == 0) return 0;
-  if (out_f64) dispatch<double>(p, A, B, C, s);
-  else dispatch<float>(p, A, B, C, s);
+  if (p.prec == TTG_FP64) {
+    if (out_f64) by_cfg<double, double>(p, A, B, C, s);
+    else by_cfg<double, float>(p, A, B, C, s);
+  } else if (p.prec == TTG_FP32) {
+    if (out_f64) by_cfg<float, double>(p, A, B, C, s);
+    else by_cfg<float, float>(p, A, B, C, s);
+  } else {
+    if (out_f64) by_cfg_bf16<double>(p, A, B, C, s);
+    else by_cfg_bf16<float>(p, A, B, C, s);
+  }
+  if (p.prec == TTG_FP64) {
+    if (out_f64) launch_reduce<double, double>(p, C, s);
+    else launch_reduce<double, float>(p, C, s);
+  } else {
+    if (out_f64) launch_reduce<float, double>(p, C, s);
+    else launch_reduce<float, float>(p, C, s);
+  }
   TTG_HIP(hipGetLastError());
   return 0;
 }

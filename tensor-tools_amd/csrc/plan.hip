@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
                                                      const uint64_t* uR, const uint64_t* uM,
                                                      QTab q, int nQ, int nblocksC, int phase,
                                                      BlkArrays ba, EmitArrays ea, PlanHdr* hdr,
-                                                     int nA, int nB) {
+                                                     int nA, int nB, int TK) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblocksC) return;
   // q group containing b: last t with qbase[t] <= b
@@ -307,6 +307,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
   double fl = 0.0;
   int64_t tw = phase == 1 ? ba.task_off[b] : 0;
   int64_t sumK = 0;
+  int64_t kt = 0;  // running k-tile count of the block's walk
   int votes[4] = {0, 0, 0, 0};
   int bad = 0;
   for (int m = q.qM0[t]; m < q.qM1[t]; m++) {
@@ -349,7 +350,9 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
     int cm = fold2(ps.nfa, em, smA, g.em0, g.sam0, g.sam1);
     int cn = fold2(ps.nfb, en, snB, g.en0, g.sbn0, g.sbn1);
     int ck = foldk(ps.np, ek, skA, skB, g.ek0, g.sak0, g.sak1, g.sbk0, g.sbk1);
-    if (ck > 2) {  // try the reversed contraction order
+    {  // reversed contraction order: taken when it folds better, or when it
+       // puts the longer component first (the GEMM walks k in chunks of the
+       // first component, so a short first component wastes chunk slots)
       int ekr[MAXR];
       int64_t skAr[MAXR], skBr[MAXR];
       for (int p = 0; p < ps.np; p++) {
@@ -357,7 +360,13 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
         skAr[p] = skA[ps.np - 1 - p];
         skBr[p] = skB[ps.np - 1 - p];
       }
-      ck = foldk(ps.np, ekr, skAr, skBr, g.ek0, g.sak0, g.sak1, g.sbk0, g.sbk1);
+      int e0r;
+      int64_t a0r, a1r, b0r, b1r;
+      const int ckr = foldk(ps.np, ekr, skAr, skBr, e0r, a0r, a1r, b0r, b1r);
+      if ((ck > 2 && ckr <= 2) || (ck == 2 && ckr == 2 && e0r > g.ek0)) {
+        ck = ckr;
+        g.ek0 = e0r; g.sak0 = a0r; g.sak1 = a1r; g.sbk0 = b0r; g.sbk1 = b1r;
+      }
     }
     if (cm > 2 || cn > 2 || ck > 2) bad = 1;
     if (phase == 0) {
@@ -366,6 +375,8 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
       votes[2] += (g.sbk0 == 1) || K == 1;
       votes[3] += (g.sbn0 == 1) || N == 1;
     } else {
+      g.kt0 = (int)kt;
+      g.pad = 0;
       ea.tasks[tw + nt] = g;
       if (ea.tuples) {
         int* tu = ea.tuples + 6 * (tw + nt);
@@ -373,19 +384,21 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
       }
     }
     nt++;
-    sumK += K;
+    kt += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK);
+    // K walk in 16-wide chunks of the first k component (gemm.hip)
+    sumK += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + 15) / 16 * 16);
     fl += 2.0 * (double)M * (double)N * (double)K;
   }
   if (phase == 0) {
     ba.ntask[b] = nt;
     ba.flops[b] = fl;
     for (int c = 0; c < TILE_NCFG; c++) {
-      const int TMc = (c == 1 || c == 3) ? 32 : (c == 4 ? 128 : 64);
-      const int TNc = (c == 1 || c == 2) ? 32 : 64;
+      const int TMc = cfg_tm(c), TNc = cfg_tn(c);
       int64_t tm = (M + TMc - 1) / TMc, tn = (N + TNc - 1) / TNc;
       atomicAdd((unsigned long long*)&hdr->tiles[c], (unsigned long long)(tm * tn));
       atomicAdd(&hdr->padded[c], (double)(tm * TMc) * (double)(tn * TNc) * (double)sumK);
     }
+    atomicMax((unsigned long long*)&hdr->kmax, (unsigned long long)sumK);
     for (int v = 0; v < 4; v++)
       if (votes[v]) atomicAdd((unsigned long long*)&hdr->votes[v], (unsigned long long)votes[v]);
     if (bad) atomicAdd((unsigned long long*)&hdr->err, 1ull);
@@ -396,6 +409,8 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
     cb.N = (int)N;
     cb.t0 = (int)tw;
     cb.t1 = (int)(tw + nt);
+    cb.nkt = (int)kt;
+    cb.pad = 0;
     ea.cblk[b] = cb;
   }
 }
@@ -432,67 +447,109 @@ __global__ void __launch_bounds__(1024) k_plan_scan(int n, const int64_t* size, 
   }
 }
 
-// ---- K6: tile list (block order, row-major tiles inside a block) ----------
-__global__ void __launch_bounds__(1024) k_plan_tiles(int n, const CBlock* cblk, int TM, int TN,
-                                                     Tile* tiles) {
-  __shared__ int64_t lds[32];
-  int64_t carry = 0;
-  for (int base = 0; base < n; base += 1024) {
-    int i = base + threadIdx.x;
-    int64_t cnt = 0, tm = 0, tn = 0;
-    if (i < n) {
-      tm = (cblk[i].M + TM - 1) / TM;
-      tn = (cblk[i].N + TN - 1) / TN;
-      cnt = tm * tn;
-    }
-    int64_t tot;
-    int64_t pos = block_excl_scan<1024>(cnt, lds, &tot);
-    if (i < n) {
-      int64_t w = carry + pos;
-      int cost = cblk[i].t1 - cblk[i].t0;
-      for (int64_t a = 0; a < tm; a++)
-        for (int64_t c = 0; c < tn; c++) {
-          Tile tl;
-          tl.c = i;
-          tl.m0 = (int)(a * TM);
-          tl.n0 = (int)(c * TN);
-          tl.cost = cost;
-          tiles[w++] = tl;
-        }
-    }
-    carry += tot;
-  }
-}
-
 }  // namespace ttg
 
 ttg_plan_s::~ttg_plan_s() {
   ttg::dfree(d_tasks);
   ttg::dfree(d_cblk);
   ttg::dfree(d_tiles);
+  ttg::dfree(d_groups);
+  ttg::dfree(d_part);
   ttg::dfree(d_tuples);
 }
 
 namespace ttg {
 
-// Tile-shape choice: estimated time = padded MFMA work / peak + tile count x
-// per-tile fixed cost (epilogue, task descriptor loads), both in units of a
-// 64x64x16 k-step.  A 128x64 tile is only chosen for large blocks.
-static int choose_cfg(const PlanHdr& h) {
+// Tile-shape autotuner (analytic).  Estimated cycles of a launch with tile
+// shape c on ncu CUs:
+//   work     = sum over tiles of k-tiles x max(MFMA cycles, staging cycles)
+//              spread over ncu CUs, plus a fixed prologue/epilogue per tile
+//              spread over 4 resident workgroups per CU;
+//   critical = the longest single tile (largest padded K walk) run alone;
+//   estimate = max(work, critical).
+// MFMA cycles per 16x16 fragment per 16-deep k-tile: FP64 256 (64-cycle
+// f64 16x16x4), FP32 128, split-BF16 24 (three 16-cycle 16x16x32 per 32 k).
+// Staging (global load, LDS store, two barriers) ~300 cycles per k-tile.
+// Split-K (build_tiles) cuts the longest walk into at most kMaxSplit pieces.
+constexpr int kMaxSplit = 16;
+constexpr int kMinPiece = 8;   // fewest k-tiles one split piece owns
+
+static int choose_cfg(const PlanHdr& h, int prec, int ncu) {
+  const double mfma = prec == TTG_FP64 ? 256.0 : (prec == TTG_FP32 ? 128.0 : 24.0);
   int best = 0;
   double bestc = 1e300;
   for (int c = 0; c < TILE_NCFG; c++) {
-    double area = (double)cfg_tm(c) * cfg_tn(c);
-    // smaller tiles run the MFMA pipe less efficiently (fewer MFMAs per
-    // LDS byte); scale their work by a penalty
-    double eff = (area >= 8192) ? 1.0 : (area >= 4096 ? 1.05 : (area >= 2048 ? 1.25 : 1.6));
-    double cost = h.padded[c] * eff / (64.0 * 64.0 * 16.0) + 6.0 * (double)h.tiles[c];
+    const double TM = cfg_tm(c), TN = cfg_tn(c);
+    const double wm = TN == 16 ? 4 : 2, wn = 4 / wm;
+    const double frags = (TM / (16 * wm)) * (TN / (16 * wn));
+    const double per_ktile = std::max(frags * mfma, 300.0);
+    const double ktiles = h.padded[c] / (TM * TN * 16.0);
+    const double work = ktiles * per_ktile / ncu + (double)h.tiles[c] * 1500.0 / (4.0 * ncu);
+    const double longest = std::max((double)h.kmax / 16.0 / kMaxSplit,
+                                    std::min((double)h.kmax / 16.0, (double)kMinPiece));
+    const double critical = longest * per_ktile + 1500.0;
+    const double cost = std::max(work, critical);
     if (cost < bestc) {
       bestc = cost;
       best = c;
     }
   }
   return best;
+}
+
+// Work list of a plan.  Every output block is cut into TM x TN tiles; a tile
+// whose k walk is longer than `piece` k-tiles is split into S <= kMaxSplit
+// consecutive k ranges, each written to its own partial slot and summed in
+// slot order by the reduction pass (deterministic).  `piece` is half of one
+// workgroup's share of the whole walk, so that the longest item is short
+// against the launch.  Items are ordered longest first; equal lengths keep
+// block order so neighbouring tiles of one block share an XCD's L2.
+int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
+  const int TM = cfg_tm(p.cfg), TN = cfg_tn(p.cfg);
+  const int64_t G = 4 * (int64_t)rt().num_cu;
+  int64_t work = 0;
+  for (const CBlock& b : cb)
+    work += (int64_t)((b.M + TM - 1) / TM) * ((b.N + TN - 1) / TN) * std::max(b.nkt, 1);
+  const int64_t piece = std::max<int64_t>(kMinPiece, (work + 2 * G - 1) / (2 * G));
+  std::vector<Tile> tiles;
+  std::vector<SplitGroup> groups;
+  int64_t nslots = 0;
+  for (size_t c = 0; c < cb.size(); c++) {
+    const CBlock& b = cb[c];
+    const int S = (int)std::min<int64_t>(kMaxSplit, std::max<int64_t>(1, (b.nkt + piece - 1) / piece));
+    for (int m0 = 0; m0 < b.M; m0 += TM)
+      for (int n0 = 0; n0 < b.N; n0 += TN) {
+        if (S == 1) {
+          tiles.push_back(Tile{(int)c, m0, n0, 0, b.nkt, -1});
+          continue;
+        }
+        groups.push_back(SplitGroup{(int)c, m0, n0, (int)nslots, S, 0});
+        for (int sp = 0; sp < S; sp++)
+          tiles.push_back(Tile{(int)c, m0, n0, (int)((int64_t)b.nkt * sp / S),
+                               (int)((int64_t)b.nkt * (sp + 1) / S), (int)(nslots + sp)});
+        nslots += S;
+      }
+  }
+  TTG_REQUIRE(tiles.size() < (size_t)INT32_MAX && nslots < INT32_MAX, "plan: work list too long");
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [](const Tile& a, const Tile& b) { return a.kt1 - a.kt0 > b.kt1 - b.kt0; });
+  hipStream_t s = stream();
+  p.ntiles = (int)tiles.size();
+  p.ngroups = (int)groups.size();
+  p.nslots = nslots;
+  p.d_tiles = dalloc<Tile>(std::max<size_t>(1, tiles.size()));
+  p.d_groups = dalloc<SplitGroup>(std::max<size_t>(1, groups.size()));
+  const size_t acc = p.prec == TTG_FP64 ? sizeof(double) : sizeof(float);
+  p.d_part = nslots ? dmalloc((size_t)nslots * TM * TN * acc) : nullptr;
+  TTG_REQUIRE(p.d_tiles && p.d_groups && (p.d_part || !nslots), "plan: work list allocation failed");
+  if (!tiles.empty())
+    TTG_HIP(hipMemcpyAsync(p.d_tiles, tiles.data(), sizeof(Tile) * tiles.size(),
+                           hipMemcpyHostToDevice, s));
+  if (!groups.empty())
+    TTG_HIP(hipMemcpyAsync(p.d_groups, groups.data(), sizeof(SplitGroup) * groups.size(),
+                           hipMemcpyHostToDevice, s));
+  TTG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
+  return 0;
 }
 
 int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
@@ -597,7 +654,7 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   EmitArrays ea{};
   const unsigned gb = (unsigned)std::max<int64_t>(1, (nblocksC + 255) / 256);
   hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
-                     uM, q, nQ, (int)nblocksC, 0, ba, ea, d_hdr, nA, nB);
+                     uM, q, nQ, (int)nblocksC, 0, ba, ea, d_hdr, nA, nB, tk_of(prec));
   hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, (int)nblocksC, ba.size, ba.ntask,
                      ba.flops, C->d_off, task_off, d_hdr);
   TTG_HIP(hipMemcpyAsync(h_hdr, d_hdr, sizeof(PlanHdr), hipMemcpyDeviceToHost, s));
@@ -622,11 +679,10 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   const int64_t nt = std::max<int64_t>(1, p->ntasks);
   p->a_mcontig = h_hdr->votes[0] >= h_hdr->votes[1];
   p->b_kcontig = h_hdr->votes[2] >= h_hdr->votes[3];
-  p->cfg = (force_cfg >= 0 && force_cfg < TILE_NCFG) ? force_cfg : choose_cfg(*h_hdr);
-  p->ntiles = (int)h_hdr->tiles[p->cfg];
+  p->cfg = (force_cfg >= 0 && force_cfg < TILE_NCFG) ? force_cfg
+                                                     : choose_cfg(*h_hdr, prec, rt().num_cu);
   p->d_tasks = dalloc<GemmTask>(nt);
   p->d_cblk = dalloc<CBlock>(std::max<int64_t>(1, nblocksC));
-  p->d_tiles = dalloc<Tile>(std::max(1, p->ntiles));
   p->d_tuples = dalloc<int>(6 * nt);
   ba.task_off = task_off;
   ba.c_off = C->d_off;
@@ -634,11 +690,17 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   ea.cblk = p->d_cblk;
   ea.tuples = p->d_tuples;
   hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
-                     uM, q, nQ, (int)nblocksC, 1, ba, ea, d_hdr, nA, nB);
-  hipLaunchKernelGGL(k_plan_tiles, dim3(1), dim3(1024), 0, s, (int)nblocksC, p->d_cblk,
-                     cfg_tm(p->cfg), cfg_tn(p->cfg), p->d_tiles);
+                     uM, q, nQ, (int)nblocksC, 1, ba, ea, d_hdr, nA, nB, tk_of(prec));
+  std::vector<CBlock> hcb(nblocksC);
+  if (nblocksC)
+    TTG_HIP(hipMemcpyAsync(hcb.data(), p->d_cblk, sizeof(CBlock) * nblocksC, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
   dfree(ba.size); dfree(ba.ntask); dfree(ba.flops); dfree(task_off);
   release();
+  if (build_tiles(*p, hcb)) {
+    delete p;
+    return 1;
+  }
   *out = p;
   return 0;
 }

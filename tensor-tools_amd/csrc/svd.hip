@@ -482,32 +482,33 @@ struct HostGemm {
   std::vector<CBlock> cblk;
 };
 
-static int run_host_gemm(const HostGemm& hg, const double* A, const double* B, double* C) {
+static int run_host_gemm(HostGemm& hg, const double* A, const double* B, double* C) {
   if (hg.cblk.empty()) return 0;
   ttg_plan_s p;
   p.prec = TTG_FP64;
   p.cfg = TILE_64x64;
-  std::vector<Tile> tiles;
-  for (size_t c = 0; c < hg.cblk.size(); c++) {
-    const CBlock& b = hg.cblk[c];
-    for (int m0 = 0; m0 < b.M; m0 += 64)
-      for (int n0 = 0; n0 < b.N; n0 += 64) tiles.push_back(Tile{(int)c, m0, n0, b.t1 - b.t0});
+  const int TK = tk_of(p.prec);
+  for (CBlock& b : hg.cblk) {   // k-tile offsets of each block's walk
+    int kt = 0;
+    for (int t = b.t0; t < b.t1; t++) {
+      GemmTask& g = hg.tasks[t];
+      g.kt0 = kt;
+      kt += (g.K / g.ek0) * ((g.ek0 + TK - 1) / TK);
+    }
+    b.nkt = kt;
   }
-  p.ntiles = (int)tiles.size();
   p.ntasks = (int64_t)hg.tasks.size();
   const GemmTask& g0 = hg.tasks[0];
   p.a_mcontig = g0.sam0 == 1;
   p.b_kcontig = g0.sbk0 == 1;
   p.d_tasks = dalloc<GemmTask>(hg.tasks.size());
   p.d_cblk = dalloc<CBlock>(hg.cblk.size());
-  p.d_tiles = dalloc<Tile>(tiles.size());
   hipStream_t s = stream();
   TTG_HIP(hipMemcpyAsync(p.d_tasks, hg.tasks.data(), sizeof(GemmTask) * hg.tasks.size(),
                          hipMemcpyHostToDevice, s));
   TTG_HIP(hipMemcpyAsync(p.d_cblk, hg.cblk.data(), sizeof(CBlock) * hg.cblk.size(),
                          hipMemcpyHostToDevice, s));
-  TTG_HIP(hipMemcpyAsync(p.d_tiles, tiles.data(), sizeof(Tile) * tiles.size(),
-                         hipMemcpyHostToDevice, s));
+  TTG_TRY(build_tiles(p, hg.cblk));
   TTG_TRY(gemm_launch(p, A, B, C, true, s));
   TTG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
   return 0;   // p's destructor releases the device arrays

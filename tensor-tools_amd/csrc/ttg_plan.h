@@ -13,29 +13,49 @@ struct GemmTask {
   int64_t sam0, sam1, sak0, sak1;
   int64_t sbk0, sbk1, sbn0, sbn1;
   int em0, ek0, en0, K;
+  int kt0, pad;       // first k-tile of this task in its block's k walk
 };
 
-// Output block c: its K-contributions are tasks [t0, t1), written once.
+// Output block c: its K-contributions are tasks [t0, t1) (nkt k-tiles in
+// total), written once.
 struct CBlock {
   int64_t c_off;
   int M, N;
   int t0, t1;
+  int nkt, pad;
 };
 
+// One workgroup work item: rows [m0, m0+TM) x cols [n0, n0+TN) of block c
+// over the k-tiles [kt0, kt1) of its walk.  slot < 0: the item owns the
+// whole walk and writes C; slot >= 0: it writes a TM x TN partial that a
+// reduction pass adds up (split-K, fixed order).
 struct Tile {
-  int c, m0, n0, cost;
+  int c, m0, n0, kt0, kt1, slot;
 };
+// reduction group of a split tile: partial slots [slot0, slot0 + ns)
+struct SplitGroup {
+  int c, m0, n0, slot0, ns, pad;
+};
+
+inline int tk_of(int prec) { return prec == TTG_BF16X3 ? 32 : 16; }
 
 struct PairSpec {
   int np, nfa, nfb;
   int pa[MAXR], pb[MAXR], fa[MAXR], fb[MAXR];
 };
 
-// tile shapes the planner chooses between (MFMA 16x16 building block)
+// tile shapes the planner chooses between (MFMA 16x16 building block); the
+// x16 shapes serve contractions whose output blocks are at most 16 wide
 enum TileCfg { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_32x64 = 3, TILE_128x64 = 4,
-               TILE_NCFG = 5 };
-inline int cfg_tm(int c) { const int t[] = {64, 32, 64, 32, 128}; return t[c]; }
-inline int cfg_tn(int c) { const int t[] = {64, 32, 32, 64, 64}; return t[c]; }
+               TILE_256x16 = 5, TILE_128x16 = 6, TILE_64x16 = 7, TILE_NCFG = 8 };
+__host__ __device__ inline int cfg_tm(int c) {
+  const int t[] = {64, 32, 64, 32, 128, 256, 128, 64};
+  return t[c];
+}
+__host__ __device__ inline int cfg_tn(int c) {
+  const int t[] = {64, 32, 32, 64, 64, 16, 16, 16};
+  return t[c];
+}
 
 // device header filled by the planning kernels
 struct PlanHdr {
@@ -43,7 +63,8 @@ struct PlanHdr {
   int64_t totalC, ntasks;
   double flops;
   int64_t tiles[TILE_NCFG];
-  double padded[TILE_NCFG];      // sum of padded tile MACs (tile area x K)
+  double padded[TILE_NCFG];      // sum over tiles of tile area x padded K
+  int64_t kmax;                  // largest padded K walk of one output block
   int64_t votes[4];              // tasks with unit stride: A m, A k, B k, B n
   int64_t err;
 };
@@ -63,6 +84,10 @@ struct ttg_plan_s {
   ttg::GemmTask* d_tasks = nullptr;
   ttg::CBlock* d_cblk = nullptr;
   ttg::Tile* d_tiles = nullptr;
+  ttg::SplitGroup* d_groups = nullptr;   // split-K reduction groups
+  int ngroups = 0;
+  void* d_part = nullptr;                // split-K partial tiles (accumulator type)
+  int64_t nslots = 0;
   // 6-tuples (c, a, b, M, N, K) for the bookkeeping checks
   int* d_tuples = nullptr;
   ~ttg_plan_s();
@@ -77,4 +102,7 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s);
 int gemm_grid(const ttg_plan_s& p);
+// work list of a plan from its output blocks (host copy): tiles of the
+// chosen shape, split along K when the tile count alone cannot fill the GPU
+int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb);
 }  // namespace ttg
