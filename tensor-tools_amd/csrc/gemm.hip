@@ -25,6 +25,7 @@
 // B[l/16][l%16]; result register r of lane l holds D[row][l%16] with
 // row = l/16 + 4r for f64 and 4*(l/16) + r for the f32-accumulating ops.
 #include <algorithm>
+#include <unordered_map>
 
 #include "ttg_internal.h"
 #include "ttg_plan.h"
@@ -393,6 +394,70 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
   }
 }
 
+// ------------------------------------------------------------------ skinny
+// Contractions whose tasks all have N <= kSkinnyN and K <= kSkinnyK (the MPO
+// stage of H_eff and of the environment update: K and N are MPO-link sector
+// dims).  One work item = kSkinnyRows rows of one output block, one row per
+// thread; per task the K x N operand block and the K offsets of A go to LDS
+// (zero-padded to kSkinnyK x kSkinnyN), each thread loads its row's K values
+// (coalesced across the wave when A's row stride is 1) and accumulates all
+// N outputs with FMA in T.  A and C are streamed once per task / once.
+template <typename T, typename OutT>
+__global__ void __launch_bounds__(kThreads)
+k_skinny(const Tile* __restrict__ tiles, const CBlock* __restrict__ cblk,
+         const GemmTask* __restrict__ tasks, const T* __restrict__ A, const T* __restrict__ B,
+         OutT* __restrict__ C) {
+  static_assert(kSkinnyRows == kThreads, "one row per thread");
+  __shared__ T Bs[kSkinnyK * kSkinnyN];
+  __shared__ int64_t koff[kSkinnyK];
+  const Tile tl = tiles[blockIdx.x];
+  const CBlock cb = cblk[tl.c];
+  const int m = tl.m0 + (int)threadIdx.x;
+  const bool row_ok = m < cb.M;
+  T acc[kSkinnyN];
+#pragma unroll
+  for (int n = 0; n < kSkinnyN; n++) acc[n] = T(0);
+  for (int t = cb.t0; t < cb.t1; t++) {
+    const GemmTask g = tasks[t];
+    __syncthreads();   // previous task's LDS reads are done
+    for (int e = threadIdx.x; e < kSkinnyK * kSkinnyN; e += kThreads) {
+      const int k = e / kSkinnyN, n = e % kSkinnyN;
+      Bs[e] = (k < g.K && n < cb.N) ? B[g.b_off + off2(k, g.ek0, g.sbk0, g.sbk1) +
+                                        off2(n, g.en0, g.sbn0, g.sbn1)]
+                                    : T(0);
+    }
+    if (threadIdx.x < kSkinnyK) {
+      const int k = threadIdx.x;
+      koff[k] = k < g.K ? off2(k, g.ek0, g.sak0, g.sak1) : 0;
+    }
+    __syncthreads();
+    if (!row_ok) continue;
+    const T* ar = A + g.a_off + off2(m, g.em0, g.sam0, g.sam1);
+    for (int k0 = 0; k0 < g.K; k0 += 8) {
+      T a[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) a[u] = k0 + u < g.K ? ar[koff[k0 + u]] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int n = 0; n < kSkinnyN; n++) acc[n] += a[u] * Bs[(k0 + u) * kSkinnyN + n];
+    }
+  }
+  if (!row_ok) return;
+  OutT* cr = C + cb.c_off + m;
+#pragma unroll
+  for (int n = 0; n < kSkinnyN; n++)
+    if (n < cb.N) cr[(int64_t)cb.M * n] = (OutT)acc[n];
+  for (int n = kSkinnyN; n < cb.N; n++) cr[(int64_t)cb.M * n] = OutT(0);   // blocks without tasks
+}
+
+template <typename T, typename OutT>
+static void launch_skinny(const ttg_plan_s& p, const void* A, const void* B, void* C,
+                          hipStream_t s) {
+  hipLaunchKernelGGL((k_skinny<T, OutT>), dim3(p.ntiles), dim3(kThreads), 0, s, p.d_tiles,
+                     p.d_cblk, p.d_tasks, (const T*)A, (const T*)B, (OutT*)C);
+}
+
 // ------------------------------------------------------------------ split-K
 // C tile of a split group = sum of its partial slots, added in slot order
 template <typename AccT, typename OutT>
@@ -420,16 +485,28 @@ static void launch_reduce(const ttg_plan_s& p, void* C, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ dispatch
-int gemm_grid(const ttg_plan_s& p) {
-  int g = rt().num_cu * 4;
-  return std::max(1, std::min(g, p.ntiles));
+// persistent grid: every workgroup the CUs can hold at once for this kernel
+// (its register / LDS occupancy), at most one per work item
+template <typename K>
+static dim3 persistent_grid(const ttg_plan_s& p, K kernel) {
+  static std::unordered_map<const void*, int> per_cu;   // by kernel instantiation
+  auto it = per_cu.find((const void*)kernel);
+  if (it == per_cu.end()) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kThreads, 0) != hipSuccess || n < 1)
+      n = 4;
+    it = per_cu.emplace((const void*)kernel, std::min(n, 8)).first;
+  }
+  const int64_t g = (int64_t)it->second * rt().num_cu;
+  return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(g, p.ntiles)));
 }
 
 template <typename T, int TM, int TN, int WM, typename OutT>
 static void launch_mode(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
-  dim3 grid(gemm_grid(p)), block(kThreads);
+  dim3 block(kThreads);
 #define TTG_L(AMv, BKv)                                                                           \
-  hipLaunchKernelGGL((k_gemm<T, TM, TN, WM, AMv, BKv, OutT>), grid, block, 0, s, p.d_tiles,       \
+  hipLaunchKernelGGL((k_gemm<T, TM, TN, WM, AMv, BKv, OutT>),                                     \
+                     persistent_grid(p, k_gemm<T, TM, TN, WM, AMv, BKv, OutT>), block, 0, s, p.d_tiles, \
                      p.ntiles, p.d_cblk, p.d_tasks, (const T*)A, (const T*)B, (OutT*)C,         \
                      (T*)p.d_part)
   if (p.a_mcontig && p.b_kcontig) TTG_L(true, true);
@@ -441,9 +518,10 @@ static void launch_mode(const ttg_plan_s& p, const void* A, const void* B, void*
 
 template <int TM, int TN, int WM, typename OutT>
 static void launch_bf16(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
-  dim3 grid(gemm_grid(p)), block(kThreads);
+  dim3 block(kThreads);
 #define TTG_L(AMv, BKv)                                                                          \
-  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>), grid, block, 0, s, p.d_tiles, \
+  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>),                                \
+                     persistent_grid(p, k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>), block, 0, s, p.d_tiles, \
                      p.ntiles, p.d_cblk, p.d_tasks, (const float*)A, (const float*)B, (OutT*)C, \
                      (float*)p.d_part)
   if (p.a_mcontig && p.b_kcontig) TTG_L(true, true);
@@ -484,6 +562,17 @@ static void by_cfg_bf16(const ttg_plan_s& p, const void* A, const void* B, void*
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s) {
   if (p.ntiles == 0) return 0;
+  if (p.cfg == TILE_SKINNY) {
+    if (p.prec == TTG_FP64) {
+      if (out_f64) launch_skinny<double, double>(p, A, B, C, s);
+      else launch_skinny<double, float>(p, A, B, C, s);
+    } else {
+      if (out_f64) launch_skinny<float, double>(p, A, B, C, s);
+      else launch_skinny<float, float>(p, A, B, C, s);
+    }
+    TTG_HIP(hipGetLastError());
+    return 0;
+  }
   if (p.prec == TTG_FP64) {
     if (out_f64) by_cfg<double, double>(p, A, B, C, s);
     else by_cfg<double, float>(p, A, B, C, s);

@@ -13,6 +13,7 @@
 // task list in the reference loop order and the grouped-GEMM tile list.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "ttg_device.h"
 #include "ttg_internal.h"
@@ -309,6 +310,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
   int64_t sumK = 0;
   int64_t kt = 0;  // running k-tile count of the block's walk
   int votes[4] = {0, 0, 0, 0};
+  int64_t ktmax = 0;
   int bad = 0;
   for (int m = q.qM0[t]; m < q.qM1[t]; m++) {
     const uint64_t mid = uM[m] & 0xffffffffull;
@@ -384,6 +386,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
       }
     }
     nt++;
+    ktmax = K > ktmax ? K : ktmax;
     kt += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK);
     // K walk in 16-wide chunks of the first k component (gemm.hip)
     sumK += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + 15) / 16 * 16);
@@ -399,6 +402,10 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
       atomicAdd(&hdr->padded[c], (double)(tm * TMc) * (double)(tn * TNc) * (double)sumK);
     }
     atomicMax((unsigned long long*)&hdr->kmax, (unsigned long long)sumK);
+    if (nt) {
+      atomicMax((unsigned long long*)&hdr->nmax, (unsigned long long)N);
+      atomicMax((unsigned long long*)&hdr->ktmax, (unsigned long long)ktmax);
+    }
     for (int v = 0; v < 4; v++)
       if (votes[v]) atomicAdd((unsigned long long*)&hdr->votes[v], (unsigned long long)votes[v]);
     if (bad) atomicAdd((unsigned long long*)&hdr->err, 1ull);
@@ -516,6 +523,10 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
   int64_t nslots = 0;
   for (size_t c = 0; c < cb.size(); c++) {
     const CBlock& b = cb[c];
+    if (p.cfg == TILE_SKINNY) {   // row chunks, whole walk, no split
+      for (int m0 = 0; m0 < b.M; m0 += TM) tiles.push_back(Tile{(int)c, m0, 0, 0, b.nkt, -1});
+      continue;
+    }
     const int S = (int)std::min<int64_t>(kMaxSplit, std::max<int64_t>(1, (b.nkt + piece - 1) / piece));
     for (int m0 = 0; m0 < b.M; m0 += TM)
       for (int n0 = 0; n0 < b.N; n0 += TN) {
@@ -679,8 +690,14 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   const int64_t nt = std::max<int64_t>(1, p->ntasks);
   p->a_mcontig = h_hdr->votes[0] >= h_hdr->votes[1];
   p->b_kcontig = h_hdr->votes[2] >= h_hdr->votes[3];
-  p->cfg = (force_cfg >= 0 && force_cfg < TILE_NCFG) ? force_cfg
-                                                     : choose_cfg(*h_hdr, prec, rt().num_cu);
+  if (force_cfg < 0) {   // TTG_TILE_CFG=<n> pins every plan's tile shape (tuning runs)
+    const char* e = getenv("TTG_TILE_CFG");
+    if (e && *e) force_cfg = atoi(e);
+  }
+  const bool skinny_ok = h_hdr->nmax <= kSkinnyN && h_hdr->ktmax <= kSkinnyK;
+  if (force_cfg == TILE_SKINNY && skinny_ok) p->cfg = TILE_SKINNY;
+  else if (force_cfg >= 0 && force_cfg < TILE_NCFG) p->cfg = force_cfg;
+  else p->cfg = skinny_ok ? TILE_SKINNY : choose_cfg(*h_hdr, prec, rt().num_cu);
   p->d_tasks = dalloc<GemmTask>(nt);
   p->d_cblk = dalloc<CBlock>(std::max<int64_t>(1, nblocksC));
   p->d_tuples = dalloc<int>(6 * nt);

@@ -45,15 +45,20 @@ struct PairSpec {
 };
 
 // tile shapes the planner chooses between (MFMA 16x16 building block); the
-// x16 shapes serve contractions whose output blocks are at most 16 wide
+// x16 shapes serve contractions whose output blocks are at most 16 wide.
+// TILE_SKINNY is the vector-ALU kernel for contractions whose every task has
+// N <= kSkinnyN and K <= kSkinnyK (one output row per thread, the K x N
+// operand block in LDS): there an MFMA tile would be mostly zero padding.
 enum TileCfg { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_32x64 = 3, TILE_128x64 = 4,
-               TILE_256x16 = 5, TILE_128x16 = 6, TILE_64x16 = 7, TILE_NCFG = 8 };
+               TILE_256x16 = 5, TILE_128x16 = 6, TILE_64x16 = 7, TILE_NCFG = 8,
+               TILE_SKINNY = 8 };
+constexpr int kSkinnyN = 16, kSkinnyK = 32, kSkinnyRows = 256;
 __host__ __device__ inline int cfg_tm(int c) {
-  const int t[] = {64, 32, 64, 32, 128, 256, 128, 64};
+  const int t[] = {64, 32, 64, 32, 128, 256, 128, 64, kSkinnyRows};
   return t[c];
 }
 __host__ __device__ inline int cfg_tn(int c) {
-  const int t[] = {64, 32, 32, 64, 64, 16, 16, 16};
+  const int t[] = {64, 32, 32, 64, 64, 16, 16, 16, kSkinnyN};
   return t[c];
 }
 
@@ -66,6 +71,7 @@ struct PlanHdr {
   double padded[TILE_NCFG];      // sum over tiles of tile area x padded K
   int64_t kmax;                  // largest padded K walk of one output block
   int64_t votes[4];              // tasks with unit stride: A m, A k, B k, B n
+  int64_t nmax, ktmax;           // widest output block with tasks; largest task K
   int64_t err;
 };
 
@@ -101,7 +107,6 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
 // a double or float C.
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s);
-int gemm_grid(const ttg_plan_s& p);
 // work list of a plan from its output blocks (host copy): tiles of the
 // chosen shape, split along K when the tile count alone cannot fill the GPU
 int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb);
