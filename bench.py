@@ -36,6 +36,99 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def stage_traffic(stage, cfg):
+    """HBM bytes per launch of one FP64 H_eff stage kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
+    each scaled by what the counter reads for a device copy of known size),
+    or None when no summary for this stage and work shape exists."""
+    path = os.path.join(ROOT, "profiles", "heff_cyl6x12_m2048_traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    rec = d.get("stages", {}).get(str(stage + 1))
+    if not rec or rec.get("work_shape") != cfg:
+        return None
+    return int(round((rec["fetch_kib"] * d["fetch_scale"] + rec["write_kib"] * d["write_scale"]) * 1024))
+
+
+def sweep_leg(P, T, args):
+    """One full two-site DMRG sweep (both half-sweeps, N-1 bonds each) on the
+    workload's lattice, max bond dimension --sweep-m, SVD cutoff 1e-9,
+    Davidson max_restart 2, tol 1e-12, from the same seeded random MPS
+    (Gaussian Sz sectors of width sigma, m per link where the Hilbert space
+    allows) in every precision form.  Reduced-precision forms end each
+    Davidson call with an FP64 product (Rayleigh quotient and residual)."""
+    from ttgpu.workloads import WORKLOADS, lattice_terms, random_mps
+    lat, Nx, Ny, yper, J2, site, m_w, smax, sigma = WORKLOADS[args.workload]
+    N = Nx * Ny
+    H = P.mpo_autompo(N, lattice_terms(P, lat, Nx, Ny, yper, J2))
+    t0 = time.time()
+    tensors = random_mps(N, args.sweep_m, sigma, seed=11)
+    log(f"[bench] sweep: random MPS N={N} m={args.sweep_m} built in {time.time() - t0:.1f}s")
+    forms = {"fp64": T.FP64, "fp32": T.FP32, "bf16x3": T.BF16X3}
+    res = {}
+    for name in args.sweep_forms.split(","):
+        P.set_precision(forms[name])
+        psi = P.mps_product([i % 2 for i in range(N)])
+        for i, t in enumerate(tensors):
+            psi.set_tensor(i, P.PTensor.from_flat(t))
+        psi.set_center(-1)
+        r = P.dmrg(psi, H, 1, [args.sweep_m], [1e-9], [2], tol=1e-12)
+        sweep_s = float(np.sum(r["halfsweep_s"]))
+        res[name] = dict(sweep_s=round(sweep_s, 4), halfsweep_s=[round(v, 4) for v in r["halfsweep_s"]],
+                         total_s=round(r["seconds"], 4), E=r["E"], E_per_site=r["E"] / N,
+                         n_matvec=r["n_matvec"], t_davidson=round(r["t_davidson"], 4),
+                         t_svd=round(r["t_svd"], 4), t_env=round(r["t_env"], 4),
+                         max_bond=int(np.max(psi.bond_dims())))
+        log(f"[bench] sweep {name}: {sweep_s:.3f} s (halves {res[name]['halfsweep_s']}, "
+            f"davidson {r['t_davidson']:.2f} s, svd {r['t_svd']:.2f} s, env {r['t_env']:.2f} s, "
+            f"{r['n_matvec']} matvecs), E/N = {r['E'] / N:.12f}, max bond {res[name]['max_bond']}")
+        del psi
+    P.set_precision(T.FP64)
+    if "fp64" in res:
+        for name, v in res.items():
+            v["dE_per_site_vs_fp64"] = abs(v["E_per_site"] - res["fp64"]["E_per_site"])
+    out = dict(m=args.sweep_m, sites=N, cutoff=1e-9, max_restart=2, forms=res)
+    if args.cpu_seconds > 0 and args.sweep_m == m_w:
+        out["cpu_baseline"] = cpu_bond_sample(args, N)
+    return out
+
+
+def cpu_bond_sample(args, N):
+    """The oracle's cost of one bulk bond update at the sweep's m (Davidson
+    with max_restart 2 on the middle-bond H_eff of the matvec workload, the
+    sector SVD with cutoff 1e-9 / K = m, one environment step), times the
+    2(N-1) bond updates of a sweep.  Bulk bonds are the largest, so this
+    over-estimates the edges of the chain."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from ttgpu import workloads as W
+    from ttgpu.flat import LINK, OUTWARD, make_index
+    W1, W2 = W.mpo_bond(O, args.workload)
+    lat, Nx, Ny, yper, J2, site, m, smax, sigma = W.WORKLOADS[args.workload]
+    L, R, x = W.heff_inputs(W1, W2, site, m, smax, sigma, 7)
+    oL, oR, oW1, oW2, ox = (O.OTensor.from_flat(t) for t in (L, R, W1, W2, x))
+    ob = O.OBig(oL, oW1, oW2, oR)
+    t0 = time.time()
+    ob.davidson(ox, 2, 1e-12)
+    t1 = time.time()
+    l, si, sj, r = x.idx
+    mid = make_index(OUTWARD, f"ID1Link{site + 1}", LINK, 0, list(zip(l.qn, l.qdim)))
+    Al, Ar = O.init_block([l, si, mid]), O.init_block([mid.dag(), sj, r])
+    O.svd_bond(ox, Al, Ar, mid.name, 0, 1e-9, m)
+    t2 = time.time()
+    O.env_step(Al, oL, oW1)
+    t3 = time.time()
+    per_bond = t3 - t0
+    log(f"[bench] cpu bond sample: davidson {t1 - t0:.2f} s, svd {t2 - t1:.2f} s, env {t3 - t2:.2f} s "
+        f"on {O.threads()} threads")
+    return {"sweep_s_estimate": round(per_bond * 2 * (N - 1), 2), "unit": "s", "cores": O.threads(),
+            "kind": "port", "per_bond_s": round(per_bond, 3),
+            "sample": "one bulk bond update (Davidson, sector SVD, environment step) of the oracle "
+                      "at the sweep's m, times 2(N-1) bonds"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +138,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of the cpu_baseline leg (0 disables)")
     ap.add_argument("--forms", default="fp64,fp32,bf16x3")
+    ap.add_argument("--sweep-m", type=int, default=2048,
+                    help="bond dimension of the DMRG sweep leg (0 skips the sweep)")
+    ap.add_argument("--sweep-forms", default="fp64,bf16x3,fp32")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a dispatch of "
+                         "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus "
@@ -56,6 +155,15 @@ def main():
     from ttgpu.workloads import WORKLOADS, workload_inputs
 
     T.init(0)
+    if args.copy_calib_mib > 0:
+        from ttgpu.flat import INWARD, LINK, FlatTensor, make_index
+        n = args.copy_calib_mib * (1 << 20) // 8
+        ix = make_index(INWARD, "calib", LINK, 0, [((0, 0), n)])
+        src = T.DTensor.from_flat(FlatTensor([ix], np.zeros((1, 1), np.int32), np.ones(n)))
+        dst = src.copy()
+        T.sync()
+        log(f"[bench] copy calibration: {n * 8} bytes read, {n * 8} bytes written")
+        del src, dst
     t0 = time.time()
     inp = workload_inputs(P, args.workload)
     W1, W2, L, R, x = (inp[k] for k in ("W1", "W2", "L", "R", "x"))
@@ -96,7 +204,8 @@ def main():
     achieved = main_form["stage_flops"][k] / (main_form["stage_ms"][k] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_PEAK_TFLOPS["f64"],
                 "unit": "TFLOP/s", "frac": round(achieved / MFMA_PEAK_TFLOPS["f64"], 4),
-                "traffic": None, "kernel": f"k_gemm stage {k + 1} (FP64 MFMA 16x16x4)"}
+                "traffic": stage_traffic(k, main_form["tile_cfg"][k]),
+                "kernel": f"stage {k + 1} grouped GEMM (FP64, work shape {main_form['tile_cfg'][k]})"}
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -126,6 +235,8 @@ def main():
         log(f"[bench] cpu baseline: {tn / n * 1e3:.1f} ms/matvec on {O.threads()} threads; "
             f"GPU FP64 vs oracle rel err {cpu['fp64_gpu_vs_oracle_relerr']:.2e}")
 
+    sweep = sweep_leg(P, T, args) if args.sweep_m > 0 else None
+
     lat, Nx, Ny, yper, J2, site, m, smax, sigma = WORKLOADS[args.workload]
     out = {
         "metric": "H_eff matvec TFLOP/s at m=2048 (FP64; split-BF16 and FP32 forms reported beside)",
@@ -148,6 +259,7 @@ def main():
                   for k, v in results.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "sweep": sweep,
     }
     if "bf16x3" in results:
         out["bf16x3_speedup_vs_fp64"] = round(results["bf16x3"]["tflops"] / main_form["tflops"], 3)

@@ -78,6 +78,10 @@ int ttg_tensor_set_random(ttg_tensor t, uint64_t seed);
  * Blocks are matched by sector tuple; x and y must share the block list
  * (as every Davidson vector does), which the call verifies. */
 int ttg_dot(ttg_tensor x, ttg_tensor y, double* out);            /* inner_product / contract */
+/* qtensor::contract for different block lists (qtensor.cpp:1095-1119): the
+ * sum over x's blocks that y also holds, matched by sector tuple; index
+ * sectors must agree, arrows may differ (the reference dags A's indices) */
+int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out);
 int ttg_nrm2(ttg_tensor x, double* out);                         /* norm */
 int ttg_scale(ttg_tensor x, double a);                           /* operator*= */
 int ttg_axpy(double a, ttg_tensor x, ttg_tensor y);              /* y.add(x, a) */
@@ -116,14 +120,15 @@ int ttg_heff_apply(ttg_heff h, ttg_tensor x, ttg_tensor y);
 /* algorithmic FLOPs of one apply (sum of 2*M*N*K over the 3 stages) and the
  * per-stage task counts */
 int ttg_heff_info(ttg_heff h, double* flops, int64_t* ntasks3, double* stage_flops3);
-/* device time of the last n applies measured with HIP events around the
- * graph launch on the library stream (ms per apply) */
+/* device time of n whole applies (x copied in, graph launch, y copied out)
+ * measured with HIP events on the library stream (ms per apply) */
 int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms_per_apply);
 /* per-stage device time (ms, mean over n runs) of the three grouped-GEMM
  * launches, HIP events on the library stream around each launch */
 int ttg_heff_stage_time(ttg_heff h, ttg_tensor x, int n, double* ms3);
-/* tile shape index (64x64, 32x32, 64x32, 32x64, 128x64) and tile count
- * the planner chose for each stage */
+/* work shape the planner chose for each stage (0..10: MFMA tiles 64x64,
+ * 32x32, 64x32, 32x64, 128x64, 256x16, 128x16, 64x16, 96x96, 96x32, 32x96;
+ * 11: the vector-ALU kernel for K <= 32, N <= 16) and its work-item count */
 int ttg_heff_tiles(ttg_heff h, int* tile_cfg3, int* ntiles3);
 int ttg_heff_free(ttg_heff h);
 

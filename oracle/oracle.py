@@ -108,6 +108,10 @@ def lib():
         L.orc_qtt_position.argtypes = [_p, _i]
         L.orc_qtt_normalize.argtypes = [_p]
         L.orc_qtt_bond_dims.argtypes = [_p, _ip]
+        L.orc_qtt_set_center.restype = None
+        L.orc_qtt_set_center.argtypes = [_p, _i]
+        L.orc_dmrg_bond_sample.restype = _i
+        L.orc_dmrg_bond_sample.argtypes = [_p, _p, _i, _d, _i, _d, _i, _i, _dp]
         L.orc_dmrg.restype = _d
         L.orc_dmrg.argtypes = [_p, _p, _i, _i, _ip, _i, _dp, _i, _ip, _d, _dp, C.POINTER(_d),
                                C.POINTER(_d)]
@@ -308,6 +312,9 @@ class OQTT:
     def center(self):
         return lib().orc_qtt_center(self.h)
 
+    def set_center(self, c: int):
+        lib().orc_qtt_set_center(self.h, c)
+
     def bond_dims(self):
         out = np.zeros(len(self) + 1, np.int32)
         lib().orc_qtt_bond_dims(self.h, out)
@@ -350,3 +357,13 @@ def dmrg(psi: OQTT, H: OQTT, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12
 
 def threads() -> int:
     return lib().orc_threads()
+
+
+def dmrg_bond_sample(psi: OQTT, H: OQTT, max_bd: int, cutoff: float, max_restart: int, tol: float,
+                     start: int, nbonds: int):
+    """(setup seconds, seconds for nbonds bond updates of a left-to-right
+    half-sweep from site `start`, last bond energy): the CPU sweep sample."""
+    out = np.zeros(3)
+    if lib().orc_dmrg_bond_sample(psi.h, H.h, max_bd, cutoff, max_restart, tol, start, nbonds, out):
+        raise RuntimeError("oracle: dmrg bond sample failed")
+    return float(out[0]), float(out[1]), float(out[2])

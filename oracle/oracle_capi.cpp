@@ -332,4 +332,42 @@ double orc_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int 
   } catch (std::exception&) { g_err = "dmrg failed"; return 0; }
 }
 
+// -1 marks the tensor train as not canonical (position() then runs rc())
+void orc_qtt_set_center(void* q, int c) { ((QTT*)q)->center = c; }
+
+// CPU timing sample of a DMRG sweep: canonicalise to site `start`,
+// normalise, build the right environments and the left ones up to `start`
+// (out[0] seconds), then nbonds bond updates of a left-to-right half-sweep
+// from `start`, each updateSite + updateEnv as in dmrg() (out[1] seconds in
+// total, out[2] the last bond energy).
+int orc_dmrg_bond_sample(void* psi_, void* H_, int max_bd, double cutoff, int max_restart,
+                         double tol, int start, int nbonds, double* out) {
+  try {
+    QTT& psi = *(QTT*)psi_;
+    QTT& H = *(QTT*)H_;
+    const int L = (int)H.length;
+    if (start < 0 || start >= L - 1) { g_err = "bond sample: bad start site"; return 1; }
+    uint64_t rng = 0x5eed;
+    auto t0 = std::chrono::steady_clock::now();
+    psi.position(start);
+    psi.normalize();
+    std::vector<QTensor> TR(L), TL(L);
+    buildEnv(psi, H, TR, TL);
+    for (int i = 0; i < start; i++) updateEnv(psi, H, TR, TL, (unsigned)i, MoveFromLeft);
+    auto t1 = std::chrono::steady_clock::now();
+    double E = 0;
+    for (int i = start; i < start + nbonds && i < L - 1; i++) {
+      SiteStats s = updateSite(psi, H, TR, TL, (unsigned)i, MoveFromLeft, max_bd, cutoff,
+                               max_restart, tol, &rng);
+      E = s.energy;
+      updateEnv(psi, H, TR, TL, (unsigned)i, MoveFromLeft);
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    out[0] = std::chrono::duration<double>(t1 - t0).count();
+    out[1] = std::chrono::duration<double>(t2 - t1).count();
+    out[2] = E;
+    return 0;
+  } catch (std::exception&) { g_err = "dmrg bond sample failed"; return 1; }
+}
+
 }  // extern "C"

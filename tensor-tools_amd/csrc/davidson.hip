@@ -93,11 +93,68 @@ __global__ void k_embed_copy(int nb, const int* map, const int64_t* soff, const 
   }
 }
 
+// sum over the blocks of x that y also holds (map: y block of each x block)
+__global__ void __launch_bounds__(256) k_dot_mapped(int nb, const int* map, const int64_t* xoff,
+                                                    const int64_t* yoff, const double* __restrict__ x,
+                                                    const double* __restrict__ y, double* part) {
+  double s = 0.0;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int yb = map[b];
+    if (yb < 0) continue;
+    const int64_t n = xoff[b + 1] - xoff[b];
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += x[xoff[b] + i] * y[yoff[yb] + i];
+  }
+  __shared__ double red[256];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(512) k_sum_parts(const double* part, int n, double* out) {
+  __shared__ double red[512];
+  double s = threadIdx.x < n ? part[threadIdx.x] : 0.0;
+  s = block_sum<512>(s, red);
+  if (threadIdx.x == 0) *out = s;
+}
+
 }  // namespace ttg
 
 using namespace ttg;
 
 extern "C" {
+
+int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out) {
+  TTG_REQUIRE(x && y && out, "dot_matched: null argument");
+  const TMeta& a = *x->m;
+  const TMeta& b = *y->m;
+  bool same = a.rank == b.rank && a.h_idx == b.h_idx;
+  for (int j = 0; same && j < a.rank; j++) same = a.nsec[j] == b.nsec[j];
+  TTG_REQUIRE(same, "dot_matched: index sectors differ");
+  hipStream_t s = stream();
+  const int nx = a.nblocks, ny = b.nblocks;
+  const int grid = std::max(1, std::min(nx, 512));
+  uint64_t* keys = dalloc<uint64_t>(std::max(ny, 1));
+  uint64_t* skeys = dalloc<uint64_t>(std::max(ny, 1));
+  int* ids = dalloc<int>(std::max(ny, 1));
+  int* sids = dalloc<int>(std::max(ny, 1));
+  int* map = dalloc<int>(std::max(nx, 1));
+  int* err = dalloc<int>(1);
+  double* part = dalloc<double>(grid + 1);
+  double r = 0.0;
+  if (nx && ny) {
+    hipLaunchKernelGGL(k_tuple_keys, dim3((ny + 255) / 256), dim3(256), 0, s, b.view(), keys, ids);
+    hipLaunchKernelGGL(k_sort_keys, dim3((ny + 255) / 256), dim3(256), 0, s, keys, ids, ny, skeys, sids);
+    hipLaunchKernelGGL(k_embed_map, dim3((nx + 255) / 256), dim3(256), 0, s, a.view(), skeys, sids,
+                       ny, b.d_off, map, err);   // blocks y lacks map to -1 (they add zero)
+    hipLaunchKernelGGL(k_dot_mapped, dim3(grid), dim3(256), 0, s, nx, map, a.d_off, b.d_off,
+                       x->d_data, y->d_data, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(512), 0, s, part, grid, part + grid);
+    TTG_HIP(hipMemcpyAsync(&r, part + grid, sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  TTG_HIP(hipStreamSynchronize(s));
+  dfree(keys); dfree(skeys); dfree(ids); dfree(sids); dfree(map); dfree(err); dfree(part);
+  *out = r;
+  return 0;
+}
 
 int ttg_small_eigh(int n, const double* M, double* evals, double* evecs) {
   TTG_TRY(ensure_init());

@@ -99,36 +99,75 @@ __device__ __forceinline__ int xcd_first_item() {
   return (G & 7) ? b : (b & 7) * (G >> 3) + (b >> 3);
 }
 
+// ------------------------------------------------------------------ operand loads
+// Raw buffer loads over one operand block: the resource's base is the block
+// start, offsets are bytes, and an offset at or above 2^31 reads as zero.
+// Row padding and the ragged end of a k chunk become such offsets, so the
+// staging loop has no branches.  Blocks are below 2^31 bytes (checked by the
+// planner), so a padding offset plus a k offset cannot wrap into range.
+constexpr unsigned kOOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7FFFFFFF,
+                                           0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 // ------------------------------------------------------------------ FP64/FP32
 // WM x (4/WM) waves; each wave owns (TM/WM) x (TN/WN) of the tile.
+//
+// Staging: element e = tid + 256 i of the TM x TK A tile (TN x TK B tile)
+// is slot i of thread tid, laid along the operand's contiguous HBM
+// direction (AM: rows, else k; BK: k, else columns) so that a wave's loads
+// coalesce.  Per task each slot keeps one 32-bit byte offset into the
+// operand block; per k-tile only the uniform (chunk, k1) offset is added.
+// LDS holds [row][k] tiles (row pitch LDK), double-buffered: one barrier per
+// k-tile.  MFMA j of a 16-deep k-tile takes tile k = 4q + j from lane group
+// q = lane / 16, so each lane reads its four k values of a row with 16-byte
+// LDS loads.
 template <typename T, int TM, int TN, int WM, bool AM, bool BK, typename OutT>
 __global__ void __launch_bounds__(kThreads)
 k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cblk,
        const GemmTask* __restrict__ tasks, const T* __restrict__ A, const T* __restrict__ B,
        OutT* __restrict__ C, T* __restrict__ part) {
   constexpr int TK = 16;
+  constexpr int LDK = sizeof(T) == 8 ? TK + 2 : TK + 4;   // 16-row bank spread
+  constexpr int ESZ = (int)sizeof(T);
   constexpr int WN = 4 / WM;
   constexpr int FM = TM / (16 * WM), FN = TN / (16 * WN);
   constexpr int NA = TM * TK / kThreads, NB = TN * TK / kThreads;
-  constexpr int LDA = TM + 1, LDB = TN + 1;
   static_assert(FM >= 1 && FN >= 1 && NA >= 1 && NB >= 1, "tile shape");
+  static_assert(TM * TK % kThreads == 0 && TN * TK % kThreads == 0, "staging slots");
   typedef typename MfmaOp<T>::acc_t acc_t;
-  __shared__ T As[TK * LDA];
-  __shared__ T Bs[TK * LDB];
+  typedef T v2t __attribute__((ext_vector_type(2)));
+  __shared__ __attribute__((aligned(16))) T As[2][TM * LDK];
+  __shared__ __attribute__((aligned(16))) T Bs[2][TN * LDK];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
-  // fixed per-thread coordinates inside the tile
-  // A: AM -> one row, NA k-rows;  !AM -> one k-row, NA rows
-  const int a_r0 = AM ? tid % TM : tid / TK;
-  const int a_k0 = AM ? tid / TM : tid % TK;
-  constexpr int a_rstep = AM ? 0 : kThreads / TK;
-  constexpr int a_kstep = AM ? kThreads / TM : 0;
-  const int b_c0 = BK ? tid / TK : tid % TN;
-  const int b_k0 = BK ? tid % TK : tid / TN;
-  constexpr int b_cstep = BK ? kThreads / TK : 0;
-  constexpr int b_kstep = BK ? 0 : kThreads / TN;
+  const int q4 = 4 * (lane >> 4), r16 = lane & 15;
+  int ar[NA], ak[NA], br[NB], bk[NB];
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    const int e = tid + kThreads * i;
+    ar[i] = AM ? e % TM : e / TK;
+    ak[i] = AM ? e / TM : e % TK;
+  }
+#pragma unroll
+  for (int i = 0; i < NB; i++) {
+    const int e = tid + kThreads * i;
+    br[i] = BK ? e / TK : e % TN;
+    bk[i] = BK ? e % TK : e / TN;
+  }
 
   for (int ti = xcd_first_item(); ti < ntiles; ti += gridDim.x) {
     const Tile tl = tiles[ti];
@@ -140,64 +179,79 @@ k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cb
       for (int j = 0; j < FN; j++) acc[i][j] = acc_t{0, 0, 0, 0};
 
     T ra[NA], rb[NB];
-    int64_t aro[NA], bco[NB];       // row / column offsets of the current task (-1: outside)
+    unsigned aoff[NA], boff[NB];
+    __amdgpu_buffer_rsrc_t rsA, rsB;
     int cur_task = -1;
     GemmTask g;
-    auto prepare = [&](int tt) {    // per-task row / column offsets
+    auto prepare = [&](int tt) {
       g = tasks[tt];
       cur_task = tt;
+      rsA = block_rsrc(A + g.a_off);
+      rsB = block_rsrc(B + g.b_off);
 #pragma unroll
       for (int i = 0; i < NA; i++) {
-        const int mg = tl.m0 + a_r0 + a_rstep * i;
-        aro[i] = mg < cb.M ? g.a_off + off2(mg, g.em0, g.sam0, g.sam1) : -1;
+        const int mg = tl.m0 + ar[i];
+        aoff[i] = mg < cb.M
+                      ? (unsigned)((off2(mg, g.em0, g.sam0, g.sam1) + (int64_t)ak[i] * g.sak0) * ESZ)
+                      : kOOB;
       }
 #pragma unroll
       for (int i = 0; i < NB; i++) {
-        const int ng = tl.n0 + b_c0 + b_cstep * i;
-        bco[i] = ng < cb.N ? g.b_off + off2(ng, g.en0, g.sbn0, g.sbn1) : -1;
+        const int ng = tl.n0 + br[i];
+        boff[i] = ng < cb.N
+                      ? (unsigned)((off2(ng, g.en0, g.sbn0, g.sbn1) + (int64_t)bk[i] * g.sbk0) * ESZ)
+                      : kOOB;
       }
     };
     auto load = [&](const KWalk& w) {
       if (w.t != cur_task) prepare(w.t);
-      const int64_t ka1 = (int64_t)w.k1 * g.sak1, kb1 = (int64_t)w.k1 * g.sbk1;
+      const unsigned ua = (unsigned)(((int64_t)w.c * g.sak0 + (int64_t)w.k1 * g.sak1) * ESZ);
+      const unsigned ub = (unsigned)(((int64_t)w.c * g.sbk0 + (int64_t)w.k1 * g.sbk1) * ESZ);
+      const int klim = g.ek0 - w.c;
 #pragma unroll
-      for (int i = 0; i < NA; i++) {
-        const int kk = w.c + a_k0 + a_kstep * i;
-        ra[i] = (aro[i] >= 0 && kk < g.ek0) ? A[aro[i] + (int64_t)kk * g.sak0 + ka1] : T(0);
-      }
+      for (int i = 0; i < NA; i++) ra[i] = buf_load<T>(rsA, ak[i] < klim ? aoff[i] + ua : kOOB);
 #pragma unroll
-      for (int i = 0; i < NB; i++) {
-        const int kk = w.c + b_k0 + b_kstep * i;
-        rb[i] = (bco[i] >= 0 && kk < g.ek0) ? B[bco[i] + (int64_t)kk * g.sbk0 + kb1] : T(0);
-      }
+      for (int i = 0; i < NB; i++) rb[i] = buf_load<T>(rsB, bk[i] < klim ? boff[i] + ub : kOOB);
     };
-    int left = tl.kt1 - tl.kt0;   // k-tiles still to stage
-    KWalk w{cb.t0, 0, 0};
+    auto store = [&](int buf) {
+#pragma unroll
+      for (int i = 0; i < NA; i++) As[buf][ar[i] * LDK + ak[i]] = ra[i];
+#pragma unroll
+      for (int i = 0; i < NB; i++) Bs[buf][br[i] * LDK + bk[i]] = rb[i];
+    };
+    int left = tl.kt1 - tl.kt0;   // k-tiles still to compute
     if (left > 0) {
-      w = kwalk_start(tasks, cb, tl.kt0, TK);
+      KWalk w = kwalk_start(tasks, cb, tl.kt0, TK);
       load(w);
-    }
-    while (left > 0) {
+      store(0);
       __syncthreads();
+      int buf = 0;
+      while (left > 0) {
+        kwalk_next(w, g, TK);
+        if (--left > 0) load(w);   // next k-tile in flight during the MFMAs
+        v2t af[FM][2], bf[FN][2];
 #pragma unroll
-      for (int i = 0; i < NA; i++) As[(a_k0 + a_kstep * i) * LDA + a_r0 + a_rstep * i] = ra[i];
+        for (int i = 0; i < FM; i++) {
+          const T* p = &As[buf][(wm * (TM / WM) + i * 16 + r16) * LDK + q4];
+          af[i][0] = *(const v2t*)p;
+          af[i][1] = *(const v2t*)(p + 2);
+        }
 #pragma unroll
-      for (int i = 0; i < NB; i++) Bs[(b_k0 + b_kstep * i) * LDB + b_c0 + b_cstep * i] = rb[i];
-      __syncthreads();
-      const int kvalid = min(TK, g.ek0 - w.c);   // uniform: skip all-zero k-steps
-      kwalk_next(w, g, TK);
-      if (--left > 0) load(w);
-      for (int k4 = 0; k4 < kvalid; k4 += 4) {
-        const int kr = k4 + (lane >> 4);
-        T af[FM], bf[FN];
+        for (int j = 0; j < FN; j++) {
+          const T* p = &Bs[buf][(wn * (TN / WN) + j * 16 + r16) * LDK + q4];
+          bf[j][0] = *(const v2t*)p;
+          bf[j][1] = *(const v2t*)(p + 2);
+        }
 #pragma unroll
-        for (int i = 0; i < FM; i++) af[i] = As[kr * LDA + wm * (TM / WM) + i * 16 + (lane & 15)];
+        for (int kk = 0; kk < 4; kk++)
 #pragma unroll
-        for (int j = 0; j < FN; j++) bf[j] = Bs[kr * LDB + wn * (TN / WN) + j * 16 + (lane & 15)];
+          for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int i = 0; i < FM; i++)
-#pragma unroll
-          for (int j = 0; j < FN; j++) acc[i][j] = MfmaOp<T>::mac(af[i], bf[j], acc[i][j]);
+            for (int j = 0; j < FN; j++)
+              acc[i][j] = MfmaOp<T>::mac(af[i][kk >> 1][kk & 1], bf[j][kk >> 1][kk & 1], acc[i][j]);
+        if (left > 0) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
       }
     }
     // epilogue: C block is column-major M x N; a split piece writes its
@@ -207,7 +261,7 @@ k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cb
     for (int i = 0; i < FM; i++)
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int nl = wn * (TN / WN) + j * 16 + (lane & 15), n = tl.n0 + nl;
+        const int nl = wn * (TN / WN) + j * 16 + r16, n = tl.n0 + nl;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int ml = wm * (TM / WM) + i * 16 + MfmaOp<T>::row(lane, r), m = tl.m0 + ml;
@@ -225,7 +279,12 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 }
 
 // Operands are FP32 in HBM, split into hi/lo bf16 planes when staged into
-// LDS (k-contiguous rows of 32, padded to 40 to spread banks).
+// LDS ([row][k] rows of 32 k, pitch 40 to spread banks), double-buffered with
+// one barrier per k-tile.  A staging slot is one row's 8-element k group;
+// per task each slot keeps a 32-bit byte offset into the operand block
+// (buffer loads, out-of-range offsets read zero as in k_gemm), per k-tile
+// the uniform chunk offset and the element's k stride are added.  MFMA
+// operands: lane l holds A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16].
 template <int TM, int TN, int WM, bool AM, bool BK, typename OutT>
 __global__ void __launch_bounds__(kThreads)
 k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cblk,
@@ -239,24 +298,29 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
   constexpr int GA = (NGA + kThreads - 1) / kThreads;
   constexpr int GB = (NGB + kThreads - 1) / kThreads;
   static_assert(FM >= 1 && FN >= 1, "tile shape");
-  __shared__ __attribute__((aligned(16))) __bf16 AsH[TM * KP];
-  __shared__ __attribute__((aligned(16))) __bf16 AsL[TM * KP];
-  __shared__ __attribute__((aligned(16))) __bf16 BsH[TN * KP];
-  __shared__ __attribute__((aligned(16))) __bf16 BsL[TN * KP];
+  __shared__ __attribute__((aligned(16))) __bf16 AsH[2][TM * KP];
+  __shared__ __attribute__((aligned(16))) __bf16 AsL[2][TM * KP];
+  __shared__ __attribute__((aligned(16))) __bf16 BsH[2][TN * KP];
+  __shared__ __attribute__((aligned(16))) __bf16 BsL[2][TN * KP];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
-  // thread -> (row, k-group) per load slot i: row-contiguous operands put
-  // lanes along the row; k-contiguous ones put 4 lanes on a row's 32 k
-  auto a_map = [&](int i, int& m, int& kg) {
+  // slot i of this thread: row / column and k group; row-contiguous operands
+  // put lanes along the rows, k-contiguous ones put 4 lanes on a row's 32 k
+  int am[GA], akg[GA], bn[GB], bkg[GB];
+#pragma unroll
+  for (int i = 0; i < GA; i++) {
     const int e = tid + kThreads * i;
-    if (AM) { m = e % TM; kg = e / TM; } else { kg = e % 4; m = e / 4; }
-  };
-  auto b_map = [&](int i, int& n, int& kg) {
+    am[i] = AM ? e % TM : e / 4;
+    akg[i] = AM ? e / TM : e % 4;
+  }
+#pragma unroll
+  for (int i = 0; i < GB; i++) {
     const int e = tid + kThreads * i;
-    if (BK) { kg = e % 4; n = e / 4; } else { n = e % TN; kg = e / TN; }
-  };
+    bn[i] = BK ? e / 4 : e % TN;
+    bkg[i] = BK ? e % 4 : e / TN;
+  }
 
   for (int ti = xcd_first_item(); ti < ntiles; ti += gridDim.x) {
     const Tile tl = tiles[ti];
@@ -267,64 +331,55 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
 #pragma unroll
       for (int j = 0; j < FN; j++) acc[i][j] = v4f{0, 0, 0, 0};
     float ra[GA][8], rb[GB][8];
-    int64_t aro[GA], bco[GB];
+    unsigned aoff[GA], boff[GB];
+    __amdgpu_buffer_rsrc_t rsA, rsB;
     int cur_task = -1;
     GemmTask g;
     auto prepare = [&](int tt) {
       g = tasks[tt];
       cur_task = tt;
+      rsA = block_rsrc(A + g.a_off);
+      rsB = block_rsrc(B + g.b_off);
 #pragma unroll
       for (int i = 0; i < GA; i++) {
-        int m, kg;
-        a_map(i, m, kg);
-        const int mg = tl.m0 + m;
-        aro[i] = (tid + kThreads * i < NGA && mg < cb.M) ? g.a_off + off2(mg, g.em0, g.sam0, g.sam1) : -1;
+        const int mg = tl.m0 + am[i];
+        aoff[i] = (tid + kThreads * i < NGA && mg < cb.M)
+                      ? (unsigned)((off2(mg, g.em0, g.sam0, g.sam1) + (int64_t)(8 * akg[i]) * g.sak0) * 4)
+                      : kOOB;
       }
 #pragma unroll
       for (int i = 0; i < GB; i++) {
-        int n, kg;
-        b_map(i, n, kg);
-        const int ng = tl.n0 + n;
-        bco[i] = (tid + kThreads * i < NGB && ng < cb.N) ? g.b_off + off2(ng, g.en0, g.sbn0, g.sbn1) : -1;
+        const int ng = tl.n0 + bn[i];
+        boff[i] = (tid + kThreads * i < NGB && ng < cb.N)
+                      ? (unsigned)((off2(ng, g.en0, g.sbn0, g.sbn1) + (int64_t)(8 * bkg[i]) * g.sbk0) * 4)
+                      : kOOB;
       }
     };
     auto load = [&](const KWalk& w) {
       if (w.t != cur_task) prepare(w.t);
-      const int64_t ka1 = (int64_t)w.k1 * g.sak1, kb1 = (int64_t)w.k1 * g.sbk1;
+      const unsigned ua = (unsigned)(((int64_t)w.c * g.sak0 + (int64_t)w.k1 * g.sak1) * 4);
+      const unsigned ub = (unsigned)(((int64_t)w.c * g.sbk0 + (int64_t)w.k1 * g.sbk1) * 4);
+      const unsigned sa = (unsigned)(g.sak0 * 4), sb = (unsigned)(g.sbk0 * 4);
+      const int klim = g.ek0 - w.c;
 #pragma unroll
       for (int i = 0; i < GA; i++) {
-        int m, kg;
-        a_map(i, m, kg);
+        const int kl = klim - 8 * akg[i];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int kk = w.c + kg * 8 + u;
-          ra[i][u] = (aro[i] >= 0 && kk < g.ek0) ? A[aro[i] + (int64_t)kk * g.sak0 + ka1] : 0.f;
-        }
+        for (int u = 0; u < 8; u++)
+          ra[i][u] = buf_load<float>(rsA, u < kl ? aoff[i] + (ua + u * sa) : kOOB);
       }
 #pragma unroll
       for (int i = 0; i < GB; i++) {
-        int n, kg;
-        b_map(i, n, kg);
+        const int kl = klim - 8 * bkg[i];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int kk = w.c + kg * 8 + u;
-          rb[i][u] = (bco[i] >= 0 && kk < g.ek0) ? B[bco[i] + (int64_t)kk * g.sbk0 + kb1] : 0.f;
-        }
+        for (int u = 0; u < 8; u++)
+          rb[i][u] = buf_load<float>(rsB, u < kl ? boff[i] + (ub + u * sb) : kOOB);
       }
     };
-    int left = tl.kt1 - tl.kt0;   // k-tiles still to stage
-    KWalk w{cb.t0, 0, 0};
-    if (left > 0) {
-      w = kwalk_start(tasks, cb, tl.kt0, TK);
-      load(w);
-    }
-    while (left > 0) {
-      __syncthreads();
+    auto store = [&](int buf) {
 #pragma unroll
       for (int i = 0; i < GA; i++) {
         if (tid + kThreads * i >= NGA) break;
-        int m, kg;
-        a_map(i, m, kg);
         v8bf h, l;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -333,14 +388,12 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
           h[u] = a;
           l[u] = b;
         }
-        *(v8bf*)&AsH[m * KP + kg * 8] = h;
-        *(v8bf*)&AsL[m * KP + kg * 8] = l;
+        *(v8bf*)&AsH[buf][am[i] * KP + akg[i] * 8] = h;
+        *(v8bf*)&AsL[buf][am[i] * KP + akg[i] * 8] = l;
       }
 #pragma unroll
       for (int i = 0; i < GB; i++) {
         if (tid + kThreads * i >= NGB) break;
-        int n, kg;
-        b_map(i, n, kg);
         v8bf h, l;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -349,34 +402,46 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
           h[u] = a;
           l[u] = b;
         }
-        *(v8bf*)&BsH[n * KP + kg * 8] = h;
-        *(v8bf*)&BsL[n * KP + kg * 8] = l;
+        *(v8bf*)&BsH[buf][bn[i] * KP + bkg[i] * 8] = h;
+        *(v8bf*)&BsL[buf][bn[i] * KP + bkg[i] * 8] = l;
       }
+    };
+    int left = tl.kt1 - tl.kt0;
+    if (left > 0) {
+      KWalk w = kwalk_start(tasks, cb, tl.kt0, TK);
+      load(w);
+      store(0);
       __syncthreads();
-      kwalk_next(w, g, TK);
-      if (--left > 0) load(w);
+      int buf = 0;
       const int kq = 8 * (lane >> 4);
-      v8bf ah[FM], al[FM], bh[FN], bl[FN];
+      while (left > 0) {
+        kwalk_next(w, g, TK);
+        if (--left > 0) load(w);
+        v8bf ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
-      for (int i = 0; i < FM; i++) {
-        const int row = wm * (TM / WM) + i * 16 + (lane & 15);
-        ah[i] = *(const v8bf*)&AsH[row * KP + kq];
-        al[i] = *(const v8bf*)&AsL[row * KP + kq];
-      }
-#pragma unroll
-      for (int j = 0; j < FN; j++) {
-        const int col = wn * (TN / WN) + j * 16 + (lane & 15);
-        bh[j] = *(const v8bf*)&BsH[col * KP + kq];
-        bl[j] = *(const v8bf*)&BsL[col * KP + kq];
-      }
-#pragma unroll
-      for (int i = 0; i < FM; i++)
+        for (int i = 0; i < FM; i++) {
+          const int row = wm * (TM / WM) + i * 16 + (lane & 15);
+          ah[i] = *(const v8bf*)&AsH[buf][row * KP + kq];
+          al[i] = *(const v8bf*)&AsL[buf][row * KP + kq];
+        }
 #pragma unroll
         for (int j = 0; j < FN; j++) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          const int col = wn * (TN / WN) + j * 16 + (lane & 15);
+          bh[j] = *(const v8bf*)&BsH[buf][col * KP + kq];
+          bl[j] = *(const v8bf*)&BsL[buf][col * KP + kq];
         }
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+          for (int j = 0; j < FN; j++) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          }
+        if (left > 0) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+      }
     }
     float* pt = tl.slot >= 0 ? part + (int64_t)tl.slot * (TM * TN) : nullptr;
 #pragma unroll
@@ -485,30 +550,76 @@ static void launch_reduce(const ttg_plan_s& p, void* C, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ dispatch
-// persistent grid: every workgroup the CUs can hold at once for this kernel
-// (its register / LDS occupancy), at most one per work item
-template <typename K>
-static dim3 persistent_grid(const ttg_plan_s& p, K kernel) {
-  static std::unordered_map<const void*, int> per_cu;   // by kernel instantiation
-  auto it = per_cu.find((const void*)kernel);
+// Tile shapes: (config, TM, TN, waves along M).
+#define TTG_TILE_SHAPES(X)                                                                       \
+  X(TILE_64x64, 64, 64, 2) X(TILE_32x32, 32, 32, 2) X(TILE_64x32, 64, 32, 2)                     \
+  X(TILE_32x64, 32, 64, 2) X(TILE_128x64, 128, 64, 2) X(TILE_256x16, 256, 16, 4)                 \
+  X(TILE_128x16, 128, 16, 4) X(TILE_64x16, 64, 16, 4) X(TILE_96x96, 96, 96, 2)                   \
+  X(TILE_96x32, 96, 32, 2) X(TILE_32x96, 32, 96, 2)
+
+// the kernel instantiation that serves (precision, shape, operand layout)
+template <typename T, int TM, int TN, int WM, typename OutT>
+static const void* fp_kernel(bool am, bool bk) {
+  if (am && bk) return (const void*)k_gemm<T, TM, TN, WM, true, true, OutT>;
+  if (am) return (const void*)k_gemm<T, TM, TN, WM, true, false, OutT>;
+  if (bk) return (const void*)k_gemm<T, TM, TN, WM, false, true, OutT>;
+  return (const void*)k_gemm<T, TM, TN, WM, false, false, OutT>;
+}
+template <int TM, int TN, int WM, typename OutT>
+static const void* bf_kernel(bool am, bool bk) {
+  if (am && bk) return (const void*)k_gemm_bf16x3<TM, TN, WM, true, true, OutT>;
+  if (am) return (const void*)k_gemm_bf16x3<TM, TN, WM, true, false, OutT>;
+  if (bk) return (const void*)k_gemm_bf16x3<TM, TN, WM, false, true, OutT>;
+  return (const void*)k_gemm_bf16x3<TM, TN, WM, false, false, OutT>;
+}
+template <typename OutT>
+static const void* kernel_of(int prec, int cfg, bool am, bool bk) {
+  switch (cfg) {
+#define TTG_K(C, TM, TN, WM)                                                                     \
+  case C:                                                                                        \
+    return prec == TTG_FP64   ? fp_kernel<double, TM, TN, WM, OutT>(am, bk)                      \
+           : prec == TTG_FP32 ? fp_kernel<float, TM, TN, WM, OutT>(am, bk)                       \
+                              : bf_kernel<TM, TN, WM, OutT>(am, bk);
+    TTG_TILE_SHAPES(TTG_K)
+#undef TTG_K
+    default: return nullptr;
+  }
+}
+
+// workgroups of a kernel one CU holds at once (register / LDS occupancy),
+// cached per instantiation
+static int occupancy_of(const void* kernel) {
+  static std::unordered_map<const void*, int> per_cu;
+  auto it = per_cu.find(kernel);
   if (it == per_cu.end()) {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kThreads, 0) != hipSuccess || n < 1)
       n = 4;
-    it = per_cu.emplace((const void*)kernel, std::min(n, 8)).first;
+    it = per_cu.emplace(kernel, std::min(n, 8)).first;
   }
-  const int64_t g = (int64_t)it->second * rt().num_cu;
+  return it->second;
+}
+
+int gemm_occupancy(int prec, int cfg, bool a_mcontig, bool b_kcontig) {
+  const void* k = prec == TTG_FP64 ? kernel_of<double>(prec, cfg, a_mcontig, b_kcontig)
+                                   : kernel_of<float>(prec, cfg, a_mcontig, b_kcontig);
+  return k ? occupancy_of(k) : 1;
+}
+
+// persistent grid: every workgroup the CUs can hold at once for this kernel,
+// at most one per work item
+static dim3 persistent_grid(const ttg_plan_s& p, const void* kernel) {
+  const int64_t g = (int64_t)occupancy_of(kernel) * rt().num_cu;
   return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(g, p.ntiles)));
 }
 
 template <typename T, int TM, int TN, int WM, typename OutT>
 static void launch_mode(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
-  dim3 block(kThreads);
+  const dim3 grid = persistent_grid(p, fp_kernel<T, TM, TN, WM, OutT>(p.a_mcontig, p.b_kcontig));
 #define TTG_L(AMv, BKv)                                                                           \
-  hipLaunchKernelGGL((k_gemm<T, TM, TN, WM, AMv, BKv, OutT>),                                     \
-                     persistent_grid(p, k_gemm<T, TM, TN, WM, AMv, BKv, OutT>), block, 0, s, p.d_tiles, \
-                     p.ntiles, p.d_cblk, p.d_tasks, (const T*)A, (const T*)B, (OutT*)C,         \
-                     (T*)p.d_part)
+  hipLaunchKernelGGL((k_gemm<T, TM, TN, WM, AMv, BKv, OutT>), grid, dim3(kThreads), 0, s,        \
+                     p.d_tiles, p.ntiles, p.d_cblk, p.d_tasks, (const T*)A, (const T*)B,        \
+                     (OutT*)C, (T*)p.d_part)
   if (p.a_mcontig && p.b_kcontig) TTG_L(true, true);
   else if (p.a_mcontig) TTG_L(true, false);
   else if (p.b_kcontig) TTG_L(false, true);
@@ -518,12 +629,11 @@ static void launch_mode(const ttg_plan_s& p, const void* A, const void* B, void*
 
 template <int TM, int TN, int WM, typename OutT>
 static void launch_bf16(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
-  dim3 block(kThreads);
+  const dim3 grid = persistent_grid(p, bf_kernel<TM, TN, WM, OutT>(p.a_mcontig, p.b_kcontig));
 #define TTG_L(AMv, BKv)                                                                          \
-  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>),                                \
-                     persistent_grid(p, k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>), block, 0, s, p.d_tiles, \
-                     p.ntiles, p.d_cblk, p.d_tasks, (const float*)A, (const float*)B, (OutT*)C, \
-                     (float*)p.d_part)
+  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>), grid, dim3(kThreads), 0, s,   \
+                     p.d_tiles, p.ntiles, p.d_cblk, p.d_tasks, (const float*)A,                \
+                     (const float*)B, (OutT*)C, (float*)p.d_part)
   if (p.a_mcontig && p.b_kcontig) TTG_L(true, true);
   else if (p.a_mcontig) TTG_L(true, false);
   else if (p.b_kcontig) TTG_L(false, true);
@@ -534,28 +644,22 @@ static void launch_bf16(const ttg_plan_s& p, const void* A, const void* B, void*
 template <typename T, typename OutT>
 static void by_cfg(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
   switch (p.cfg) {
-    case TILE_64x64: launch_mode<T, 64, 64, 2, OutT>(p, A, B, C, s); break;
-    case TILE_32x32: launch_mode<T, 32, 32, 2, OutT>(p, A, B, C, s); break;
-    case TILE_64x32: launch_mode<T, 64, 32, 2, OutT>(p, A, B, C, s); break;
-    case TILE_32x64: launch_mode<T, 32, 64, 2, OutT>(p, A, B, C, s); break;
-    case TILE_128x64: launch_mode<T, 128, 64, 2, OutT>(p, A, B, C, s); break;
-    case TILE_256x16: launch_mode<T, 256, 16, 4, OutT>(p, A, B, C, s); break;
-    case TILE_128x16: launch_mode<T, 128, 16, 4, OutT>(p, A, B, C, s); break;
-    default: launch_mode<T, 64, 16, 4, OutT>(p, A, B, C, s); break;
+#define TTG_C(C_, TM, TN, WM) \
+  case C_: launch_mode<T, TM, TN, WM, OutT>(p, A, B, C, s); break;
+    TTG_TILE_SHAPES(TTG_C)
+#undef TTG_C
+    default: break;
   }
 }
 
 template <typename OutT>
 static void by_cfg_bf16(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
   switch (p.cfg) {
-    case TILE_64x64: launch_bf16<64, 64, 2, OutT>(p, A, B, C, s); break;
-    case TILE_32x32: launch_bf16<32, 32, 2, OutT>(p, A, B, C, s); break;
-    case TILE_64x32: launch_bf16<64, 32, 2, OutT>(p, A, B, C, s); break;
-    case TILE_32x64: launch_bf16<32, 64, 2, OutT>(p, A, B, C, s); break;
-    case TILE_128x64: launch_bf16<128, 64, 2, OutT>(p, A, B, C, s); break;
-    case TILE_256x16: launch_bf16<256, 16, 4, OutT>(p, A, B, C, s); break;
-    case TILE_128x16: launch_bf16<128, 16, 4, OutT>(p, A, B, C, s); break;
-    default: launch_bf16<64, 16, 4, OutT>(p, A, B, C, s); break;
+#define TTG_C(C_, TM, TN, WM) \
+  case C_: launch_bf16<TM, TN, WM, OutT>(p, A, B, C, s); break;
+    TTG_TILE_SHAPES(TTG_C)
+#undef TTG_C
+    default: break;
   }
 }
 

@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
   int64_t sumK = 0;
   int64_t kt = 0;  // running k-tile count of the block's walk
   int votes[4] = {0, 0, 0, 0};
-  int64_t ktmax = 0;
+  int64_t ktmax = 0, opmax = 0;
   int bad = 0;
   for (int m = q.qM0[t]; m < q.qM1[t]; m++) {
     const uint64_t mid = uM[m] & 0xffffffffull;
@@ -387,9 +387,11 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
     }
     nt++;
     ktmax = K > ktmax ? K : ktmax;
+    opmax = M * K > opmax ? M * K : opmax;
+    opmax = K * N > opmax ? K * N : opmax;
     kt += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK);
-    // K walk in 16-wide chunks of the first k component (gemm.hip)
-    sumK += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + 15) / 16 * 16);
+    // K walk in TK-wide chunks of the first k component (gemm.hip)
+    sumK += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK * TK);
     fl += 2.0 * (double)M * (double)N * (double)K;
   }
   if (phase == 0) {
@@ -405,6 +407,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
     if (nt) {
       atomicMax((unsigned long long*)&hdr->nmax, (unsigned long long)N);
       atomicMax((unsigned long long*)&hdr->ktmax, (unsigned long long)ktmax);
+      atomicMax((unsigned long long*)&hdr->opmax, (unsigned long long)opmax);
     }
     for (int v = 0; v < 4; v++)
       if (votes[v]) atomicAdd((unsigned long long*)&hdr->votes[v], (unsigned long long)votes[v]);
@@ -467,34 +470,52 @@ ttg_plan_s::~ttg_plan_s() {
 
 namespace ttg {
 
-// Tile-shape autotuner (analytic).  Estimated cycles of a launch with tile
-// shape c on ncu CUs:
-//   work     = sum over tiles of k-tiles x max(MFMA cycles, staging cycles)
-//              spread over ncu CUs, plus a fixed prologue/epilogue per tile
-//              spread over 4 resident workgroups per CU;
-//   critical = the longest single tile (largest padded K walk) run alone;
-//   estimate = max(work, critical).
-// MFMA cycles per 16x16 fragment per 16-deep k-tile: FP64 256 (64-cycle
-// f64 16x16x4), FP32 128, split-BF16 24 (three 16-cycle 16x16x32 per 32 k).
-// Staging (global load, LDS store, two barriers) ~300 cycles per k-tile.
+// Tile-shape autotuner (analytic).  For tile shape c a workgroup's k-tile
+// costs, per wave (one wave per SIMD):
+//   mfma  = fragments x MFMA cycles per k-tile (FP64 f64 16x16x4: 64 cycles,
+//           4 per 16 k; FP32 f32 16x16x4: 32; split-BF16: three 16-cycle
+//           16x16x32 per 32 k),
+//   issue = 4 cycles x the staging / address / LDS instructions of the tile's
+//           load slots (about 6 per FP64/FP32 element, 8 per split-BF16
+//           element including the hi/lo split) plus fragment loads and loop,
+//   lat   = global-load latency (kLatency) shared by the occ workgroups a CU
+//           keeps resident (occupancy of the actual kernel instantiation).
+// A CU retires one workgroup k-tile per max(mfma, issue, lat / occ) cycles;
+// each work item adds its first load and its epilogue.  The launch takes
+// max(total work over the CUs, the longest item after split-K run at the
+// single-workgroup rate max(mfma, issue, lat)).
 // Split-K (build_tiles) cuts the longest walk into at most kMaxSplit pieces.
 constexpr int kMaxSplit = 16;
 constexpr int kMinPiece = 8;   // fewest k-tiles one split piece owns
+constexpr double kLatency = 2000.0;
 
-static int choose_cfg(const PlanHdr& h, int prec, int ncu) {
-  const double mfma = prec == TTG_FP64 ? 256.0 : (prec == TTG_FP32 ? 128.0 : 24.0);
+static int choose_cfg(const PlanHdr& h, int prec, int ncu, bool am, bool bk) {
+  const int TK = tk_of(prec);
   int best = 0;
   double bestc = 1e300;
   for (int c = 0; c < TILE_NCFG; c++) {
     const double TM = cfg_tm(c), TN = cfg_tn(c);
     const double wm = TN == 16 ? 4 : 2, wn = 4 / wm;
-    const double frags = (TM / (16 * wm)) * (TN / (16 * wn));
-    const double per_ktile = std::max(frags * mfma, 300.0);
-    const double ktiles = h.padded[c] / (TM * TN * 16.0);
-    const double work = ktiles * per_ktile / ncu + (double)h.tiles[c] * 1500.0 / (4.0 * ncu);
-    const double longest = std::max((double)h.kmax / 16.0 / kMaxSplit,
-                                    std::min((double)h.kmax / 16.0, (double)kMinPiece));
-    const double critical = longest * per_ktile + 1500.0;
+    const double fm = TM / (16 * wm), fn = TN / (16 * wn);
+    double mfma, issue;
+    if (prec == TTG_BF16X3) {
+      mfma = fm * fn * 3 * 16;
+      const double slots = std::ceil(TM * TK / 8 / 256.0) + std::ceil(TN * TK / 8 / 256.0);
+      issue = 4.0 * (slots * 8 * 8 + 4 * (fm + fn) + 24);
+    } else {
+      mfma = fm * fn * (TK / 4) * (prec == TTG_FP64 ? 64.0 : 32.0);
+      const double slots = (TM + TN) * TK / 256.0;
+      issue = 4.0 * (6 * slots + 2 * (fm + fn) + 24);
+    }
+    const double occ = gemm_occupancy(prec, c, am, bk);
+    const double per_ktile = std::max(std::max(mfma, issue), kLatency / occ);
+    const double single = std::max(std::max(mfma, issue), kLatency);
+    const double ktiles = h.padded[c] / (TM * TN * (double)TK);
+    const double item = kLatency + TM * TN / 256.0 * 16.0;   // first load + epilogue stores
+    const double work = (ktiles * per_ktile + (double)h.tiles[c] * item / occ) / ncu;
+    const double kwalk = (double)h.kmax / TK;
+    const double longest = std::max(kwalk / kMaxSplit, std::min(kwalk, (double)kMinPiece));
+    const double critical = longest * single + item;
     const double cost = std::max(work, critical);
     if (cost < bestc) {
       bestc = cost;
@@ -677,6 +698,12 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
     return 1;
   }
   C->total = (size_t)h_hdr->totalC;
+  if (h_hdr->opmax >= (int64_t(1) << 28)) {   // the GEMM's 32-bit byte offsets per block
+    dfree(ba.size); dfree(ba.ntask); dfree(ba.flops); dfree(task_off);
+    release();
+    set_error("plan: an operand block exceeds 2^28 elements");
+    return 1;
+  }
 
   ttg_plan p = new ttg_plan_s();
   p->A = A;
@@ -697,7 +724,8 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   const bool skinny_ok = h_hdr->nmax <= kSkinnyN && h_hdr->ktmax <= kSkinnyK;
   if (force_cfg == TILE_SKINNY && skinny_ok) p->cfg = TILE_SKINNY;
   else if (force_cfg >= 0 && force_cfg < TILE_NCFG) p->cfg = force_cfg;
-  else p->cfg = skinny_ok ? TILE_SKINNY : choose_cfg(*h_hdr, prec, rt().num_cu);
+  else p->cfg = skinny_ok ? TILE_SKINNY
+                          : choose_cfg(*h_hdr, prec, rt().num_cu, p->a_mcontig, p->b_kcontig);
   p->d_tasks = dalloc<GemmTask>(nt);
   p->d_cblk = dalloc<CBlock>(std::max<int64_t>(1, nblocksC));
   p->d_tuples = dalloc<int>(6 * nt);

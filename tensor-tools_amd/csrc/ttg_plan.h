@@ -49,16 +49,18 @@ struct PairSpec {
 // TILE_SKINNY is the vector-ALU kernel for contractions whose every task has
 // N <= kSkinnyN and K <= kSkinnyK (one output row per thread, the K x N
 // operand block in LDS): there an MFMA tile would be mostly zero padding.
+// The 96-wide shapes fit sector dims up to 96 with one tile per dimension.
 enum TileCfg { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_32x64 = 3, TILE_128x64 = 4,
-               TILE_256x16 = 5, TILE_128x16 = 6, TILE_64x16 = 7, TILE_NCFG = 8,
-               TILE_SKINNY = 8 };
+               TILE_256x16 = 5, TILE_128x16 = 6, TILE_64x16 = 7, TILE_96x96 = 8, TILE_96x32 = 9,
+               TILE_32x96 = 10, TILE_NCFG = 11,
+               TILE_SKINNY = 11 };
 constexpr int kSkinnyN = 16, kSkinnyK = 32, kSkinnyRows = 256;
 __host__ __device__ inline int cfg_tm(int c) {
-  const int t[] = {64, 32, 64, 32, 128, 256, 128, 64, kSkinnyRows};
+  const int t[] = {64, 32, 64, 32, 128, 256, 128, 64, 96, 96, 32, kSkinnyRows};
   return t[c];
 }
 __host__ __device__ inline int cfg_tn(int c) {
-  const int t[] = {64, 32, 32, 64, 64, 16, 16, 16, kSkinnyN};
+  const int t[] = {64, 32, 32, 64, 64, 16, 16, 16, 96, 32, 96, kSkinnyN};
   return t[c];
 }
 
@@ -72,6 +74,7 @@ struct PlanHdr {
   int64_t kmax;                  // largest padded K walk of one output block
   int64_t votes[4];              // tasks with unit stride: A m, A k, B k, B n
   int64_t nmax, ktmax;           // widest output block with tasks; largest task K
+  int64_t opmax;                 // largest operand block (elements) of any task
   int64_t err;
 };
 
@@ -107,6 +110,9 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
 // a double or float C.
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s);
+// workgroups per CU the GEMM kernel of (precision, tile shape, operand
+// layout) can keep resident
+int gemm_occupancy(int prec, int cfg, bool a_mcontig, bool b_kcontig);
 // work list of a plan from its output blocks (host copy): tiles of the
 // chosen shape, split along K when the tile count alone cannot fill the GPU
 int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb);

@@ -262,6 +262,8 @@ void* tth_qtt_tensor(void* q, int i) { TTH_GUARD(nullptr, return new Q(((qMPS<do
 int tth_qtt_set_tensor(void* q, int i, void* t) { TTH_GUARD(-1, ((qMPS<double>*)q)->A.at(i) = *(Q*)t; return 0;) }
 int tth_qtt_length(void* q) { return (int)((qMPS<double>*)q)->length; }
 int tth_qtt_center(void* q) { return ((qMPS<double>*)q)->center; }
+// -1 marks the tensor train as not canonical (the next position() runs rc())
+void tth_qtt_set_center(void* q, int c) { ((qMPS<double>*)q)->center = c; }
 void tth_mps_free(void* q) { delete (qMPS<double>*)q; }
 void tth_mpo_free(void* q) { delete (qMPO<double>*)q; }
 double tth_mps_position(void* q, int site) { TTH_GUARD(kNaN, return ((qMPS<double>*)q)->position(site);) }
@@ -269,6 +271,14 @@ int tth_mps_normalize(void* q) { TTH_GUARD(-1, ((qMPS<double>*)q)->normalize(); 
 void tth_qtt_bond_dims(void* q, int* out) {
   auto* p = (qMPS<double>*)q;
   for (size_t i = 0; i < p->bond_dims.size(); i++) out[i] = (int)p->bond_dims[i];
+}
+
+// mps/observables.cpp:64-105, 415-450
+double tth_psiHphi(void* psi, void* H, void* phi) {
+  TTH_GUARD(kNaN, return psiHphi(*(qMPS<double>*)psi, *(qMPO<double>*)H, *(qMPS<double>*)phi);)
+}
+double tth_psiphi(void* psi, void* phi) {
+  TTH_GUARD(kNaN, return psiphi(*(qMPS<double>*)psi, *(qMPS<double>*)phi);)
 }
 
 // dmrg.cpp:760-813.  stats: [t_davidson, t_svd, t_env, n_matvec, seconds]

@@ -119,8 +119,29 @@ double tensor_davidsonIT(big_qtensor<double>& A, qtensor<double>& x_in, int m, i
     }
   }
   x.normalize();
+  // Reduced-precision forms: one FP64 product gives the Rayleigh quotient
+  // and residual of the final vector in FP64.  The quotient's error is
+  // second order in the vector's error, so the FP32-level accuracy of the
+  // reduced-precision iteration becomes an energy accurate to ~1e-12 relative.
+  const double lambda_reduced = lambda;
+  bool corrected = false;
+  if (heff_precision() != TTG_FP64) {
+    qtensor<double> y;
+    A.product_into_fp64(x, y);
+    nmv++;
+    lambda = y.inner_product(x);
+    y.add(x, -lambda);
+    qnorm = y.norm();
+    corrected = true;
+  }
   x_in = std::move(x);
-  if (st) { st->n_matvec = nmv; st->iterations = ii; st->residual = qnorm; }
+  if (st) {
+    st->n_matvec = nmv;
+    st->iterations = ii;
+    st->residual = qnorm;
+    st->fp64_correction = corrected;
+    st->lambda_reduced = lambda_reduced;
+  }
   return lambda;
 }
 

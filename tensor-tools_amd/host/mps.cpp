@@ -608,6 +608,70 @@ void updateSite(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>
 }
 
 // dmrg.cpp:760-813
+// ---------------------------------------------------------------- observables
+// mps/observables.cpp:64-105.  Left to right: acc <- ((acc * psi_i^+') * W_i) * phi_i,
+// closed by all-ones end tensors on the open boundary links.
+double psiHphi(qMPS<double>& psi, qMPO<double>& H, qMPS<double>& phi) {
+  if (psi.length != phi.length || psi.length != H.length || psi.length == 0)
+    fail("psiHphi: chain lengths differ");
+  qtensor<double> acc;
+  double res = 0;
+  for (unsigned i = 0; i < psi.length; i++) {
+    qtensor<double> t1 = psi.A[i];
+    t1.dag();
+    t1.prime();
+    qtensor<double>& t2 = H.A[i];
+    qtensor<double>& t3 = phi.A[i];
+    if (i == 0) {
+      qtensor<double> le({t1.idx_set[0], t2.idx_set[0], t3.idx_set[0]});
+      le.initBlock();
+      le.setOne();
+      le.dag();
+      acc = ((le * t1) * t2) * t3;
+    } else {
+      acc = ((acc * t1) * t2) * t3;
+    }
+    if (i + 1 == psi.length) {
+      qtensor<double> re({t1.idx_set.back(), t2.idx_set.back(), t3.idx_set.back()});
+      re.initBlock();
+      re.setOne();
+      re.dag();
+      res = acc.contract(re);
+    }
+  }
+  return res;
+}
+
+// mps/observables.cpp:415-450 (links of psi primed, sites shared)
+double psiphi(qMPS<double>& psi, qMPS<double>& phi) {
+  if (psi.length != phi.length || psi.length == 0) fail("psiphi: chain lengths differ");
+  qtensor<double> acc;
+  double res = 0;
+  for (unsigned i = 0; i < psi.length; i++) {
+    qtensor<double> t1 = psi.A[i];
+    t1.dag();
+    t1.primeLink();
+    qtensor<double>& t2 = phi.A[i];
+    if (i == 0) {
+      qtensor<double> le({t1.idx_set[0], t2.idx_set[0]});
+      le.initBlock();
+      le.setOne();
+      le.dag();
+      acc = (le * t1) * t2;
+    } else {
+      acc = (t1 * acc) * t2;
+    }
+    if (i + 1 == psi.length) {
+      qtensor<double> re({t1.idx_set.back(), t2.idx_set.back()});
+      re.initBlock();
+      re.setOne();
+      re.dag();
+      res = acc.contract(re);
+    }
+  }
+  return res;
+}
+
 double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart, double tol,
             DmrgLog* log) {

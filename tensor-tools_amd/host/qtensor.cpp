@@ -201,9 +201,13 @@ qtensor<double>& qtensor<double>::operator*=(double c) {
   return *this;
 }
 
+// qtensor.cpp:1095-1119: blocks matched by key, missing ones add nothing
 double qtensor<double>::contract(const qtensor& A) const {
   double r = 0;
-  check(ttg_dot(dev.get(), A.dev.get(), &r), "contract");
+  int same = 0;
+  check(ttg_same_blocks(dev.get(), A.dev.get(), &same), "contract");
+  if (same) check(ttg_dot(dev.get(), A.dev.get(), &r), "contract");
+  else check(ttg_dot_matched(dev.get(), A.dev.get(), &r), "contract");
   return r;
 }
 double qtensor<double>::inner_product(const qtensor& A) const { return contract(A); }
@@ -233,7 +237,9 @@ big_qtensor<double>::~big_qtensor() { drop(); }
 
 void big_qtensor<double>::drop() {
   if (heff) ttg_heff_free(heff);
+  if (heff64) ttg_heff_free(heff64);
   heff = nullptr;
+  heff64 = nullptr;
   full = qtensor<double>();
 }
 
@@ -275,6 +281,10 @@ void big_qtensor<double>::ensure_plan(const qtensor<double>& v) {
   check(ttg_heff_create(L->dev.get(), mid.dev.get(), R->dev.get(), full.dev.get(), enc.data(),
                         g_heff_prec, &heff),
         "heff_create");
+  if (g_heff_prec != TTG_FP64)
+    check(ttg_heff_create(L->dev.get(), mid.dev.get(), R->dev.get(), full.dev.get(), enc.data(),
+                          TTG_FP64, &heff64),
+          "heff_create (FP64 correction)");
   plan_prec = g_heff_prec;
 }
 
@@ -303,6 +313,24 @@ void big_qtensor<double>::product_into(qtensor<double>& v, qtensor<double>& y) {
     qtensor<double> w = widen(v);
     check(ttg_heff_apply(heff, w.dev.get(), y.dev.get()), "heff apply");
   }
+}
+
+void big_qtensor<double>::product_into_fp64(qtensor<double>& v, qtensor<double>& y) {
+  ensure_plan(v);
+  if (!heff64) {
+    product_into(v, y);
+    return;
+  }
+  if (!y.dev || !same_blocks(y, full)) {
+    ttg_tensor t;
+    check(ttg_heff_new_output(heff64, &t), "heff output");
+    y.dev = adopt(t);
+    y.idx_set = v.idx_set;
+    y.rank = v.rank;
+    y._initted = true;
+  }
+  qtensor<double> w = same_blocks(v, full) ? v : widen(v);
+  check(ttg_heff_apply(heff64, w.dev.get(), y.dev.get()), "heff apply (FP64)");
 }
 
 // big_qtensor.cpp:23-40 (prime(-1) of the result is implicit: y takes x's indices)

@@ -171,6 +171,9 @@ class big_qtensor<double> {
   void addMid(qtensor<double>* m);             // big_qtensor.h:56
   qtensor<double> product(qtensor<double>& v); // big_qtensor.cpp:23-40
   void product_into(qtensor<double>& v, qtensor<double>& y);
+  // y = H v in FP64 whatever the operator's precision form (the residual
+  // correction of the reduced-precision forms)
+  void product_into_fp64(qtensor<double>& v, qtensor<double>& y);
   double expec(qtensor<double>& v);            // big_qtensor.cpp:45-64
   size_t size() const;                         // big_qtensor.cpp:96-118
   double flops_per_product() const;
@@ -186,13 +189,22 @@ class big_qtensor<double> {
   qtensor<double> mid;
   mutable size_t size_ = 0;
   ttg_heff heff = nullptr;
+  ttg_heff heff64 = nullptr;                    // FP64 twin of a reduced-precision heff
   qtensor<double> full;                         // layout the operator was planned for
   int plan_prec = -1;
 };
 bool same_blocks(const qtensor<double>& a, const qtensor<double>& b);
 
 // linalg/tensor_davidson.cpp:192-347
-struct DavidsonStats { int n_matvec = 0; int iterations = 0; double residual = 0; };
+// n_matvec counts every product, the FP64 correction included; residual is
+// the final ||H x - lambda x|| (FP64 products when the correction ran)
+struct DavidsonStats {
+  int n_matvec = 0;
+  int iterations = 0;
+  double residual = 0;
+  bool fp64_correction = false;
+  double lambda_reduced = 0;   // the reduced-precision Ritz value before the correction
+};
 double tensor_davidsonIT(big_qtensor<double>& A, qtensor<double>& x, int m, int max_restart,
                          double tol, char mode, DavidsonStats* st = nullptr);
 
@@ -243,6 +255,10 @@ std::vector<std::pair<int, int>> square_lattice(int Nx, int Ny, bool yperiodic,
                                                 std::vector<int>* type, bool nnn);
 std::vector<SpinTerm> heisenberg_terms(const std::vector<std::pair<int, int>>& bonds,
                                        const std::vector<double>& J);
+
+// mps/observables.cpp: <psi|H|phi> (:64-105) and <psi|phi> (:415-450)
+double psiHphi(qMPS<double>& psi, qMPO<double>& H, qMPS<double>& phi);
+double psiphi(qMPS<double>& psi, qMPS<double>& phi);
 
 // algos/dmrg/dmrg.cpp
 void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,

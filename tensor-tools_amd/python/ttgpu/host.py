@@ -47,6 +47,8 @@ def _lib():
             "tth_mpo_tensor": ([_p, _i], _p), "tth_mpo_length": ([_p], _i),
             "tth_qtt_tensor": ([_p, _i], _p), "tth_qtt_set_tensor": ([_p, _i, _p], _i),
             "tth_qtt_length": ([_p], _i), "tth_qtt_center": ([_p], _i),
+            "tth_qtt_set_center": ([_p, _i], None),
+            "tth_psiHphi": ([_p, _p, _p], _d), "tth_psiphi": ([_p, _p], _d),
             "tth_mps_free": ([_p], None), "tth_mpo_free": ([_p], None),
             "tth_mps_position": ([_p, _i], _d), "tth_mps_normalize": ([_p], _i),
             "tth_qtt_bond_dims": ([_p, _ip], None),
@@ -252,6 +254,10 @@ class PMPS:
     def center(self):
         return _lib().tth_qtt_center(self.h)
 
+    def set_center(self, c: int):
+        """-1: not canonical (the next position() right-canonicalises first)"""
+        _lib().tth_qtt_set_center(self.h, c)
+
     def bond_dims(self):
         out = np.zeros(len(self) + 1, np.int32)
         _lib().tth_qtt_bond_dims(self.h, out)
@@ -278,6 +284,16 @@ def mpo_heisenberg_nn(N: int, mpo_id: int = 2) -> PMPO:
 def mps_product(state, mps_id: int = 1) -> PMPS:
     st = np.ascontiguousarray(np.array(state, dtype=np.int32))
     return PMPS(_lib().tth_mps_product(len(st), mps_id, st))
+
+
+def psiHphi(psi: PMPS, H: PMPO, phi: PMPS) -> float:
+    """<psi|H|phi> (mps/observables.cpp:64-105)"""
+    return _ok(_lib().tth_psiHphi(psi.h, H.h, phi.h), "psiHphi")
+
+
+def psiphi(psi: PMPS, phi: PMPS) -> float:
+    """<psi|phi> (mps/observables.cpp:415-450)"""
+    return _ok(_lib().tth_psiphi(psi.h, phi.h), "psiphi")
 
 
 def dmrg(psi: PMPS, H: PMPO, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12):

@@ -61,6 +61,7 @@ def load(path: str = LIB_PATH):
         "ttg_tensor_copy": ([_p, C.POINTER(_p)], _i), "ttg_tensor_free": ([_p], _i),
         "ttg_tensor_fill": ([_p, _d], _i), "ttg_tensor_set_random": ([_p, C.c_uint64], _i),
         "ttg_dot": ([_p, _p, C.POINTER(_d)], _i), "ttg_nrm2": ([_p, C.POINTER(_d)], _i),
+        "ttg_dot_matched": ([_p, _p, C.POINTER(_d)], _i),
         "ttg_scale": ([_p, _d], _i), "ttg_axpy": ([_d, _p, _p], _i),
         "ttg_contract_plan": ([_p, _p, _i, _ip, _ip, _i, C.POINTER(_p)], _i),
         "ttg_plan_info": ([_p, C.POINTER(_i), C.POINTER(_i64), C.POINTER(_d)], _i),
@@ -176,6 +177,12 @@ class DTensor:
     def dot(self, other) -> float:
         out = _d(0)
         check(lib().ttg_dot(self.h, other.h, C.byref(out)))
+        return out.value
+
+    def dot_matched(self, other) -> float:
+        """qtensor::contract over the blocks both tensors hold"""
+        out = _d(0)
+        check(lib().ttg_dot_matched(self.h, other.h, C.byref(out)))
         return out.value
 
     def norm(self) -> float:

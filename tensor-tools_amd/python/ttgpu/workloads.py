@@ -81,6 +81,37 @@ def heff_inputs(W1: FlatTensor, W2: FlatTensor, site: int, m: int, smax: int, si
     return L, R, x
 
 
+def mps_link_sectors(N: int, b: int, m: int, sigma: float):
+    """Sectors ((q0, 0), dim) of link b of an N-site spin-1/2 MPS with total
+    Sz = 0 (q0 = 2 Sz of the b sites left of the link, the product-state
+    convention of qtt.cpp:213-240): q0 runs over the charges both sides can
+    reach; dims follow exp(-Sz^2 / (2 sigma^2)) scaled to m (minimum 1) and
+    are capped by the number of left and right basis states of that charge."""
+    from math import comb
+    qs = [q for q in range(-b, b + 1, 2) if abs(q) <= N - b]
+    w = np.exp(-(np.array(qs, dtype=float) / 2.0) ** 2 / (2 * sigma * sigma))
+    d = np.maximum(1, np.round(m * w / w.sum())).astype(int)
+    out = []
+    for q, dd in zip(qs, d):
+        cap = min(comb(b, (b + q) // 2), comb(N - b, (N - b + q) // 2))
+        out.append((int(q), int(min(int(dd), cap))))
+    return out
+
+
+def random_mps(N: int, m: int, sigma: float, seed: int, mps_id: int = 1):
+    """Site tensors psi_i = (ID<id>Link<i> In, Site<i> In, ID<id>Link<i+1> Out)
+    (qtt.cpp:246-253) over all legal blocks, N(0,1) from PCG64(seed + i)."""
+    out = []
+    for i in range(N):
+        li = make_index(INWARD, f"ID{mps_id}Link{i}", LINK, 0,
+                        [((q, 0), d) for q, d in mps_link_sectors(N, i, m, sigma)])
+        si = site_index(INWARD, f"Site{i}", 0)
+        lo = make_index(OUTWARD, f"ID{mps_id}Link{i + 1}", LINK, 0,
+                        [((q, 0), d) for q, d in mps_link_sectors(N, i + 1, m, sigma)])
+        out.append(fill([li, si, lo], seed + i))
+    return out
+
+
 def lattice_terms(builder, lattice: str, Nx: int, Ny: int, yper: bool, J2: float):
     kind = 1 if lattice == "j1j2" else 0
     return builder.heisenberg_terms(kind, Nx, Ny, yper, 1.0, J2)

@@ -15,7 +15,7 @@ from ttgpu.flat import INWARD, LINK, OUTWARD, SITE, make_index, random_tensor
 pytestmark = pytest.mark.gpu
 
 TOL = {T.FP64: 1e-12, T.FP32: 2e-6, T.BF16X3: 2e-5}
-NCFG = 9   # 8 MFMA tile shapes + the skinny kernel
+NCFG = 12   # 11 MFMA tile shapes + the skinny kernel
 
 
 def sec(qs, ds):
