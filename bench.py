@@ -68,6 +68,7 @@ def sweep_leg(P, T, args):
     log(f"[bench] sweep: random MPS N={N} m={args.sweep_m} built in {time.time() - t0:.1f}s")
     forms = {"fp64": T.FP64, "fp32": T.FP32, "bf16x3": T.BF16X3}
     res = {}
+    P.set_dmrg_progress(12)
     for name in args.sweep_forms.split(","):
         P.set_precision(forms[name])
         psi = P.mps_product([i % 2 for i in range(N)])

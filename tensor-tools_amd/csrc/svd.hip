@@ -196,48 +196,110 @@ __global__ void k_svd_gather(MetaView A, int nleft, const double* __restrict__ a
 
 // 4. parallel cyclic Jacobi, one workgroup per sector.  n even (padded with
 // a zero row/column when odd; the pad never rotates).  Packed upper triangle
-// (i <= j at j(j+1)/2 + i) in LDS when it fits, else in HBM scratch.
+// (i <= j at j(j+1)/2 + i).  Dynamic LDS holds, when they fit: the triangle,
+// the eigenvector matrix V and the table of 2x2 block pairs of a round;
+// otherwise V, then also the triangle, live in HBM (scratch / output).
+// A sweep converges when no rotation is significant: |a_pq| above 1e-15 of
+// sqrt(a_pp a_qq) and above 1e-14 of the largest diagonal.  Below that floor
+// a rotation moves no eigenvalue by more than 1e-14 of the largest (singular
+// values below 1e-7 of the largest, under the Gram matrix's own resolution of
+// ~sqrt(eps)), and the rounding noise in the null space of a rank-deficient
+// sector's Gram matrix never passes the relative test alone.
 constexpr int kJacThreads = 1024;
-constexpr int kLdsDoubles = 18000;   // 141 KB of packed triangle (+12 KB static)
+constexpr int kLdsDoubles = 20000;   // dynamic LDS (160 000 B of the CU's 160 KiB)
 
 __device__ inline int pk(int i, int j) { return i <= j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j; }
+
+// LDS placement of one sector of size n (doubles): rotation arrays always;
+// then the packed triangle, V and the pair table in that priority.  Bit 0 of
+// *where: triangle in LDS, bit 1: V in LDS, bit 2: pair table in LDS.
+struct JacLayout {
+  int64_t rot, tri, v, tab, total;
+  int where;
+};
+__host__ __device__ inline JacLayout jac_layout(int n) {
+  const int64_t np = n + (n & 1), npair = np / 2;
+  JacLayout L;
+  L.rot = 3 * npair;   // c, s (doubles) and the (p, q) pairs (two ints)
+  L.tri = np * (np + 1) / 2;
+  L.v = (int64_t)n * n;
+  L.tab = (npair * (npair + 1) / 2 + 1) / 2;
+  L.where = 0;
+  int64_t used = L.rot;
+  if (used + L.tri <= kLdsDoubles) { used += L.tri; L.where |= 1; }
+  if (used + L.v <= kLdsDoubles) { used += L.v; L.where |= 2; }
+  if (used + L.tab <= kLdsDoubles) { used += L.tab; L.where |= 4; }
+  L.total = used;
+  return L;
+}
+
+__device__ inline void tri_index(int64_t w, int& i, int& j) {   // w -> (i <= j)
+  j = (int)((sqrt(8.0 * (double)w + 1.0) - 1.0) / 2.0);
+  while ((int64_t)j * (j + 1) / 2 > w) j--;
+  while ((int64_t)(j + 1) * (j + 2) / 2 <= w) j++;
+  i = (int)(w - (int64_t)j * (j + 1) / 2);
+}
 
 __global__ void __launch_bounds__(kJacThreads)
 k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
          const double* __restrict__ Gin, double* __restrict__ gscratch,
          const int64_t* __restrict__ poff, double* __restrict__ Vout,
-         double* __restrict__ evals, const int64_t* __restrict__ eoff, int max_sweeps) {
+         double* __restrict__ evals, const int64_t* __restrict__ eoff, int max_sweeps,
+         int nmax_here) {
   extern __shared__ double lds_g[];
-  __shared__ double cs_c[512], cs_s[512];
-  __shared__ int pp[512], qq[512];
   __shared__ int any_rot;
+  __shared__ double red[kJacThreads / 64];
   const int q = blockIdx.x;
   const int n = nq[q];                 // true size
-  if (n == 0) return;
+  if (n == 0 || n > nmax_here) return;   // larger sectors: block Jacobi
   const int np = n + (n & 1);          // padded even size
   const int npair = np / 2;
   const int64_t psize = (int64_t)np * (np + 1) / 2;
-  double* g = (psize <= kLdsDoubles) ? lds_g : gscratch + poff[q];
+  const int nblk = npair * (npair + 1) / 2;
+  const JacLayout lay = jac_layout(n);
+  double* cs_c = lds_g;
+  double* cs_s = lds_g + npair;
+  int* pp = (int*)(lds_g + 2 * npair);
+  int* qq = pp + npair;
+  double* next = lds_g + lay.rot;
+  double* g = (lay.where & 1) ? next : gscratch + poff[q];
+  if (lay.where & 1) next += psize;
+  double* Vg = Vout + goff[q];         // n x n column-major eigenvectors (output)
+  double* V = (lay.where & 2) ? next : Vg;
+  if (lay.where & 2) next += (int64_t)n * n;
+  int* tab = (lay.where & 4) ? (int*)next : nullptr;
   const double* G0 = Gin + goff[q];    // full n x n column-major
-  double* V = Vout + goff[q];          // n x n column-major eigenvectors
-  // load packed triangle (pad row/col zero), V = I
-  for (int64_t e = threadIdx.x; e < psize; e += kJacThreads) {
-    int j = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
-    while ((int64_t)j * (j + 1) / 2 > e) j--;
-    while ((int64_t)(j + 1) * (j + 2) / 2 <= e) j++;
-    int i = (int)(e - (int64_t)j * (j + 1) / 2);
+  // load packed triangle (pad row/col zero), V = I, block-pair table
+  for (int64_t e = threadIdx.x; e < psize; e += blockDim.x) {
+    int i, j;
+    tri_index(e, i, j);
     g[e] = (i < n && j < n) ? 0.5 * (G0[i + (int64_t)n * j] + G0[j + (int64_t)n * i]) : 0.0;
   }
-  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kJacThreads)
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x)
     V[e] = (e % n == e / n) ? 1.0 : 0.0;
+  if (tab)
+    for (int w = threadIdx.x; w < nblk; w += blockDim.x) {
+      int k1, k2;
+      tri_index(w, k1, k2);
+      tab[w] = k1 | (k2 << 16);
+    }
+  // largest diagonal element: the scale of the convergence floor
+  double dmax = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    dmax = fmax(dmax, fabs(G0[i + (int64_t)n * i]));
+  for (int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dmax;
   __syncthreads();
+  dmax = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); i++) dmax = fmax(dmax, red[i]);
+  const double floor_abs = 1e-14 * dmax;
   const int m1 = np - 1;
   for (int sweep = 0; sweep < max_sweeps; sweep++) {
     if (threadIdx.x == 0) any_rot = 0;
     __syncthreads();
     for (int round = 0; round < m1; round++) {
       // circle method: player m1 fixed, pairs (round, m1) and (round+k, round-k)
-      for (int k = threadIdx.x; k < npair; k += kJacThreads) {
+      for (int k = threadIdx.x; k < npair; k += blockDim.x) {
         int a, b;
         if (k == 0) { a = round; b = m1; }
         else { a = (round + k) % m1; b = (round - k + m1) % m1; }
@@ -247,7 +309,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
         const double app = g[pk(a, a)], aqq = g[pk(b, b)], apq = g[pk(a, b)];
         double c = 1.0, s = 0.0;
         if (apq != 0.0) {
-          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) any_rot = 1;
+          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > floor_abs) any_rot = 1;
           const double theta = (aqq - app) / (2.0 * apq);
           const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
           c = 1.0 / sqrt(t * t + 1.0);
@@ -258,12 +320,10 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
       }
       __syncthreads();
       // G <- J^T G J over 2x2 blocks (k1 <= k2) of the packed triangle
-      const int nblk = npair * (npair + 1) / 2;
-      for (int w = threadIdx.x; w < nblk; w += kJacThreads) {
-        int k2 = (int)((sqrt(8.0 * (double)w + 1.0) - 1.0) / 2.0);
-        while (k2 * (k2 + 1) / 2 > w) k2--;
-        while ((k2 + 1) * (k2 + 2) / 2 <= w) k2++;
-        const int k1 = w - k2 * (k2 + 1) / 2;
+      for (int w = threadIdx.x; w < nblk; w += blockDim.x) {
+        int k1, k2;
+        if (tab) { k1 = tab[w] & 0xffff; k2 = tab[w] >> 16; }
+        else tri_index(w, k1, k2);
         const int a = pp[k1], b = qq[k1], c2i = pp[k2], d = qq[k2];
         const double c1 = cs_c[k1], s1 = cs_s[k1], c2 = cs_c[k2], s2 = cs_s[k2];
         if (k1 == k2) {
@@ -286,7 +346,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
         }
       }
       // V <- V J (columns of real indices only)
-      for (int w = threadIdx.x; w < npair * n; w += kJacThreads) {
+      for (int w = threadIdx.x; w < npair * n; w += blockDim.x) {
         const int k = w / n, i = w % n;
         const int a = pp[k], b = qq[k];
         if (b >= n) continue;   // pad never rotates (apq == 0)
@@ -300,7 +360,310 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
     __syncthreads();
     if (!any_rot) break;
   }
-  for (int i = threadIdx.x; i < n; i += kJacThreads) evals[eoff[q] + i] = g[pk(i, i)];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) evals[eoff[q] + i] = g[pk(i, i)];
+  if (lay.where & 2)
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) Vg[e] = V[e];
+}
+
+// 4b. Sectors larger than kJacBlockMin: two-sided block Jacobi.  The padded
+// Gram matrix (N = n rounded up to 64, zero padding never rotates) is cut into
+// 32-wide blocks; a round pairs the blocks (circle method) and
+//   k_bjac_solve  diagonalises every 64 x 64 subproblem G[I+J, I+J] in LDS
+//                 (one inner cyclic sweep, one wave per pair) into Q_IJ, and
+//   k_bjac_apply  updates G <- P^T G P and V <- V P, P = diag(Q_IJ), as
+//                 64 x 64 tile products,
+// so a round moves G and V through memory once instead of once per scalar
+// rotation.  A sweep (nbk - 1 rounds) leaves a sector active while any
+// subproblem held an off-diagonal element above the scalar kernel's
+// threshold (1e-15 sqrt(g_ii g_jj) and 1e-14 of the largest diagonal).
+constexpr int kJacBlockMin = 64;
+constexpr int kBJ = 32;                // block width; subproblems are 2 kBJ
+constexpr int kSub = 2 * kBJ;
+constexpr int kSubLd = kSub + 1;       // LDS row pitch (bank spread)
+
+struct BJSector {
+  int n, N, nbk, q;                    // size, padded size, blocks, sector id
+  int64_t gp, qp;                      // offsets of padded G / V and of the pair Q's
+  double dmax;
+};
+
+// index (0..63) of subproblem entry t for the pair k of round r
+__device__ inline int bj_pair_block(int nbk, int r, int k, int half) {
+  const int m1 = nbk - 1;
+  int a, b;
+  if (k == 0) { a = r % m1; b = m1; }
+  else { a = (r + k) % m1; b = (r - k + m1) % m1; }
+  if (a > b) { int t = a; a = b; b = t; }
+  return half ? b : a;
+}
+__device__ inline int bj_index(int nbk, int r, int k, int t) {
+  return bj_pair_block(nbk, r, k, t >= kBJ) * kBJ + (t & (kBJ - 1));
+}
+
+__global__ void k_bjac_init(const BJSector* sec, int nsec, const int64_t* goff, const double* G,
+                            double* Gp, double* Vp) {
+  const int li = blockIdx.y;
+  if (li >= nsec) return;
+  const BJSector S = sec[li];
+  const double* g = G + goff[S.q];
+  const int64_t NN = (int64_t)S.N * S.N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < NN;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % S.N), j = (int)(e / S.N);
+    Gp[S.gp + e] = (i < S.n && j < S.n) ? 0.5 * (g[i + (int64_t)S.n * j] + g[j + (int64_t)S.n * i]) : 0.0;
+    Vp[S.gp + e] = i == j ? 1.0 : 0.0;
+  }
+}
+
+// one workgroup per (sector, pair): the 64 x 64 subproblem, one cyclic
+// sweep; each round's 32 disjoint rotations are spread over the 256 threads
+// as (rotation, element) pairs
+constexpr int kSolveThreads = 256;
+__global__ void __launch_bounds__(kSolveThreads)
+k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const int* active, int* flag,
+             const double* Gp, double* Qp) {
+  __shared__ double Ss[kSub * kSubLd];
+  __shared__ double Qs[kSub * kSubLd];
+  __shared__ double cs_c[kBJ], cs_s[kBJ];
+  __shared__ int pa[kBJ], pb[kBJ];
+  __shared__ int sig;
+  const int2 it = items[blockIdx.x];
+  const int li = it.x, k = it.y;
+  const BJSector S = sec[li];
+  if (!active[li] || r >= S.nbk - 1) return;
+  const int tid = threadIdx.x;
+  if (tid == 0) sig = 0;
+  for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
+    const int i = e % kSub, j = e / kSub;
+    const int gi = bj_index(S.nbk, r, k, i), gj = bj_index(S.nbk, r, k, j);
+    Ss[i * kSubLd + j] = Gp[S.gp + gi + (int64_t)S.N * gj];
+    Qs[i * kSubLd + j] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  // significance test of this subproblem (before rotating)
+  bool any = false;
+  for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
+    const int i = e % kSub, j = e / kSub;
+    if (i == j) continue;
+    const double v = Ss[i * kSubLd + j];
+    const double dd = sqrt(fabs(Ss[i * kSubLd + i] * Ss[j * kSubLd + j]));
+    any |= fabs(v) > 1e-15 * dd && fabs(v) > 1e-14 * S.dmax;
+  }
+  if (any) sig = 1;
+  __syncthreads();
+  double* Q = Qp + S.qp + (int64_t)k * kSub * kSub;   // column-major 64 x 64
+  if (!sig) {   // already diagonal: Q = I
+    for (int e = tid; e < kSub * kSub; e += kSolveThreads) Q[e] = (e % kSub == e / kSub) ? 1.0 : 0.0;
+    return;
+  }
+  if (tid == 0) atomicOr(&flag[li], 1);
+  const int m1 = kSub - 1;
+  for (int round = 0; round < m1; round++) {
+    if (tid < kBJ) {
+      int a, b;
+      if (tid == 0) { a = round; b = m1; }
+      else { a = (round + tid) % m1; b = (round - tid + m1) % m1; }
+      if (a > b) { int t = a; a = b; b = t; }
+      pa[tid] = a;
+      pb[tid] = b;
+      const double app = Ss[a * kSubLd + a], aqq = Ss[b * kSubLd + b], apq = Ss[a * kSubLd + b];
+      double c = 1.0, sn = 0.0;
+      if (apq != 0.0) {
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        c = 1.0 / sqrt(tt * tt + 1.0);
+        sn = tt * c;
+      }
+      cs_c[tid] = c;
+      cs_s[tid] = sn;
+    }
+    __syncthreads();
+    // rows a, b of S
+    for (int e = tid; e < kBJ * kSub; e += kSolveThreads) {
+      const int p = e / kSub, j = e % kSub;
+      const int a = pa[p], b = pb[p];
+      const double c = cs_c[p], sn = cs_s[p];
+      const double xa = Ss[a * kSubLd + j], xb = Ss[b * kSubLd + j];
+      Ss[a * kSubLd + j] = c * xa - sn * xb;
+      Ss[b * kSubLd + j] = sn * xa + c * xb;
+    }
+    __syncthreads();
+    // columns a, b of S and Q
+    for (int e = tid; e < kBJ * kSub; e += kSolveThreads) {
+      const int p = e / kSub, i = e % kSub;
+      const int a = pa[p], b = pb[p];
+      const double c = cs_c[p], sn = cs_s[p];
+      const double xa = Ss[i * kSubLd + a], xb = Ss[i * kSubLd + b];
+      Ss[i * kSubLd + a] = c * xa - sn * xb;
+      Ss[i * kSubLd + b] = sn * xa + c * xb;
+      const double qa = Qs[i * kSubLd + a], qb = Qs[i * kSubLd + b];
+      Qs[i * kSubLd + a] = c * qa - sn * qb;
+      Qs[i * kSubLd + b] = sn * qa + c * qb;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
+    const int i = e % kSub, j = e / kSub;
+    Q[e] = Qs[i * kSubLd + j];
+  }
+}
+
+// tile products of a round: kind 0: G[Ia, Ib] <- Qa^T G[Ia, Ib] Qb;
+// kind 1: V[rows r0..r0+63, Ib] <- V[.., Ib] Qb
+__global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const int4* items, int nitems,
+                                                    int r, const int* active, double* Gp, double* Vp,
+                                                    const double* Qp) {
+  __shared__ double Ts[kSub * kSubLd];
+  __shared__ double Qa[kSub * kSubLd];
+  __shared__ double Qb[kSub * kSubLd];
+  const int4 it = items[blockIdx.x];
+  const int li = it.x;
+  const BJSector S = sec[li];
+  if (!active[li] || r >= S.nbk - 1) return;
+  const int kind = it.w, a = it.y, b = it.z;
+  const int tid = threadIdx.x;
+  const double* qa = Qp + S.qp + (int64_t)a * kSub * kSub;
+  const double* qb = Qp + S.qp + (int64_t)b * kSub * kSub;
+  double* M = kind == 0 ? Gp : Vp;
+  // load: rows (kind 0: pair a's indices; kind 1: a*64 + t), columns: pair b's indices
+  for (int e = tid; e < kSub * kSub; e += 256) {
+    const int i = e % kSub, j = e / kSub;
+    const int gi = kind == 0 ? bj_index(S.nbk, r, a, i) : a * kSub + i;
+    const int gj = bj_index(S.nbk, r, b, j);
+    Ts[i * kSubLd + j] = M[S.gp + gi + (int64_t)S.N * gj];
+    Qb[i * kSubLd + j] = qb[i + kSub * j];
+    if (kind == 0) Qa[i * kSubLd + j] = qa[i + kSub * j];
+  }
+  __syncthreads();
+  // U = T Qb: thread owns rows i0..i0+3, columns j0..j0+3
+  const int i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
+  double u[4][4];
+  for (int x = 0; x < 4; x++)
+    for (int y = 0; y < 4; y++) u[x][y] = 0.0;
+  for (int kk = 0; kk < kSub; kk++) {
+    double tv[4], qv[4];
+    for (int x = 0; x < 4; x++) tv[x] = Ts[(i0 + x) * kSubLd + kk];
+    for (int y = 0; y < 4; y++) qv[y] = Qb[kk * kSubLd + j0 + y];
+    for (int x = 0; x < 4; x++)
+      for (int y = 0; y < 4; y++) u[x][y] += tv[x] * qv[y];
+  }
+  if (kind == 0) {   // R = Qa^T U
+    __syncthreads();
+    for (int x = 0; x < 4; x++)
+      for (int y = 0; y < 4; y++) Ts[(i0 + x) * kSubLd + j0 + y] = u[x][y];
+    __syncthreads();
+    for (int x = 0; x < 4; x++)
+      for (int y = 0; y < 4; y++) u[x][y] = 0.0;
+    for (int kk = 0; kk < kSub; kk++) {
+      double qv[4], tv[4];
+      for (int x = 0; x < 4; x++) qv[x] = Qa[kk * kSubLd + i0 + x];
+      for (int y = 0; y < 4; y++) tv[y] = Ts[kk * kSubLd + j0 + y];
+      for (int x = 0; x < 4; x++)
+        for (int y = 0; y < 4; y++) u[x][y] += qv[x] * tv[y];
+    }
+  }
+  for (int x = 0; x < 4; x++)
+    for (int y = 0; y < 4; y++) {
+      const int gi = kind == 0 ? bj_index(S.nbk, r, a, i0 + x) : a * kSub + i0 + x;
+      const int gj = bj_index(S.nbk, r, b, j0 + y);
+      M[S.gp + gi + (int64_t)S.N * gj] = u[x][y];
+    }
+}
+
+__global__ void k_bjac_finish(const BJSector* sec, int nsec, const int64_t* goff,
+                              const int64_t* eoff, const double* Gp, const double* Vp,
+                              double* Vout, double* evals) {
+  const int li = blockIdx.y;
+  if (li >= nsec) return;
+  const BJSector S = sec[li];
+  const int64_t nn = (int64_t)S.n * S.n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nn;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % S.n), j = (int)(e / S.n);
+    Vout[goff[S.q] + e] = Vp[S.gp + i + (int64_t)S.N * j];
+    if (i == j) evals[eoff[S.q] + i] = Gp[S.gp + i + (int64_t)S.N * i];
+  }
+}
+
+// host driver of the block Jacobi over the sectors `big` (sizes nq)
+static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
+                        const std::vector<int64_t>& goff_h, const int64_t* d_goff,
+                        const int64_t* d_eoff, const double* G, double* Vev, double* ev,
+                        hipStream_t s, int max_sweeps) {
+  const int ns = (int)big.size();
+  std::vector<BJSector> sec(ns);
+  std::vector<int2> pitems;
+  std::vector<int4> titems;
+  int64_t gp = 0, qp = 0;
+  int maxr = 0;
+  std::vector<double> dmax(ns, 0.0);
+  for (int li = 0; li < ns; li++) {
+    const int q = big[li], n = nq[q];
+    const int N = (n + kSub - 1) / kSub * kSub;
+    const int nbk = N / kBJ, npair = nbk / 2;
+    sec[li] = BJSector{n, N, nbk, q, gp, qp, 0.0};
+    gp += (int64_t)N * N;
+    qp += (int64_t)npair * kSub * kSub;
+    maxr = std::max(maxr, nbk - 1);
+    for (int k = 0; k < npair; k++) pitems.push_back(int2{li, k});
+    for (int a = 0; a < npair; a++)
+      for (int b = 0; b < npair; b++) titems.push_back(int4{li, a, b, 0});
+    for (int rc = 0; rc < N / kSub; rc++)
+      for (int b = 0; b < npair; b++) titems.push_back(int4{li, rc, b, 1});
+  }
+  // largest diagonal per sector (host copy of the Gram diagonals)
+  for (int li = 0; li < ns; li++) {
+    const int q = big[li], n = nq[q];
+    std::vector<double> d(n);
+    TTG_HIP(hipMemcpy2DAsync(d.data(), sizeof(double), G + goff_h[q], sizeof(double) * (n + 1),
+                             sizeof(double), n, hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipStreamSynchronize(s));
+    for (double v : d) dmax[li] = std::max(dmax[li], std::fabs(v));
+    sec[li].dmax = dmax[li];
+  }
+  BJSector* d_sec = dalloc<BJSector>(ns);
+  int2* d_pit = dalloc<int2>(pitems.size());
+  int4* d_tit = dalloc<int4>(titems.size());
+  double* Gp = dalloc<double>(gp);
+  double* Vp = dalloc<double>(gp);
+  double* Qp = dalloc<double>(qp);
+  int* act = dalloc<int>(ns);
+  int* flag = dalloc<int>(ns);
+  auto release = [&]() {
+    dfree(d_sec); dfree(d_pit); dfree(d_tit); dfree(Gp); dfree(Vp); dfree(Qp); dfree(act); dfree(flag);
+  };
+  if (!d_sec || !d_pit || !d_tit || !Gp || !Vp || !Qp || !act || !flag) {
+    release();
+    set_error("svd: block Jacobi allocation failed");
+    return 1;
+  }
+  std::vector<int> hact(ns, 1);
+  TTG_HIP(hipMemcpyAsync(d_sec, sec.data(), sizeof(BJSector) * ns, hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(d_pit, pitems.data(), sizeof(int2) * pitems.size(), hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(d_tit, titems.data(), sizeof(int4) * titems.size(), hipMemcpyHostToDevice, s));
+  TTG_HIP(hipMemcpyAsync(act, hact.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_bjac_init, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, G, Gp, Vp);
+  for (int sweep = 0; sweep < max_sweeps; sweep++) {
+    TTG_HIP(hipMemsetAsync(flag, 0, sizeof(int) * ns, s));
+    for (int r = 0; r < maxr; r++) {
+      hipLaunchKernelGGL(k_bjac_solve, dim3((unsigned)pitems.size()), dim3(kSolveThreads), 0, s, d_sec, d_pit,
+                         (int)pitems.size(), r, act, flag, Gp, Qp);
+      hipLaunchKernelGGL(k_bjac_apply, dim3((unsigned)titems.size()), dim3(256), 0, s, d_sec, d_tit,
+                         (int)titems.size(), r, act, Gp, Vp, Qp);
+    }
+    std::vector<int> hf(ns);
+    TTG_HIP(hipMemcpyAsync(hf.data(), flag, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipStreamSynchronize(s));
+    int any = 0;
+    for (int li = 0; li < ns; li++) any |= hf[li];
+    if (!any) break;
+    TTG_HIP(hipMemcpyAsync(act, hf.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
+  }
+  hipLaunchKernelGGL(k_bjac_finish, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, d_eoff, Gp, Vp,
+                     Vev, ev);
+  TTG_HIP(hipStreamSynchronize(s));
+  release();
+  return 0;
 }
 
 // 5a. per sector: eigen order (descending), top-k singular values in the
@@ -332,32 +695,40 @@ __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq
 // 5b. truncation (qtensor_op.cpp:1280-1301): nm and cumsum are summed
 // sequentially in the reference's order; the global order is a stable
 // descending sort (the reference's std::sort is unstable on ties).
-__global__ void __launch_bounds__(256) k_svd_truncate(int T, const double* svals, const int* ssec,
-                                                      int* order, double cutoff, int64_t K,
-                                                      int* newdim, double* out) {
-  for (int i = threadIdx.x; i < T; i += blockDim.x) {
-    const double v = svals[i];
-    int r = 0;
-    for (int j = 0; j < T; j++) r += (svals[j] > v) || (svals[j] == v && j < i);
-    order[r] = i;
+// 5b. global descending order of all singular values (ties by position,
+// a stable sort as the reference's pairCompare sort over (sector, desc) order)
+__global__ void __launch_bounds__(256) k_svd_rank(int T, const double* __restrict__ svals,
+                                                  int* __restrict__ order, double* __restrict__ sorted) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T) return;
+  const double v = svals[i];
+  int r = 0;
+  for (int j = 0; j < T; j++) r += (svals[j] > v) || (svals[j] == v && j < i);
+  order[r] = i;
+  sorted[r] = v;
+}
+
+// 5c. the reference's keep loop (qtensor_op.cpp:1294-1300), sequential in its
+// summation order: nm over (sector, descending) order, cumsum over the sorted
+// values
+__global__ void k_svd_truncate(int T, const double* svals, const int* ssec, const int* order,
+                               const double* sorted, double cutoff, int64_t K, int* newdim,
+                               double* out) {
+  if (threadIdx.x != 0) return;
+  double nm = 0.0;
+  for (int i = 0; i < T; i++) nm += svals[i] * svals[i];
+  double cumsum = 0.0;
+  int64_t acc = 0;
+  for (int i = 0; i < T; i++) {
+    const double s = sorted[i];
+    cumsum += s * s / nm;
+    if (s == 0. && cutoff != 0.) break;
+    newdim[ssec[order[i]]] += 1;
+    acc += 1;
+    if ((1 - cumsum < cutoff && cutoff != 0.) || (acc >= K && K != 0)) break;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double nm = 0.0;
-    for (int i = 0; i < T; i++) nm += svals[i] * svals[i];
-    double cumsum = 0.0;
-    int64_t acc = 0;
-    for (int i = 0; i < T; i++) {
-      const double s = svals[order[i]];
-      cumsum += s * s / nm;
-      if (s == 0. && cutoff != 0.) break;
-      newdim[ssec[order[i]]] += 1;
-      acc += 1;
-      if ((1 - cumsum < cutoff && cutoff != 0.) || (acc >= K && K != 0)) break;
-    }
-    out[0] = fmax(1 - cumsum, 0.0);
-    out[1] = (double)acc;
-  }
+  out[0] = fmax(1 - cumsum, 0.0);
+  out[1] = (double)acc;
 }
 
 // 6a. block lists of U (left..., mid) and V (mid, right...) and S (mid)
@@ -588,7 +959,6 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   const bool fromLeft = direction == TTG_MOVE_FROM_LEFT;
   std::vector<int64_t> moff(nQ + 1, 0), goff(nQ + 1, 0), poff(nQ + 1, 0), eoff(nQ + 1, 0);
   std::vector<int> nq(nQ), kq(nQ);
-  int64_t maxpsize = 0;
   for (int q = 0; q < nQ; q++) {
     const int64_t n = fromLeft ? rows[q] : cols[q];
     TTG_REQUIRE(n <= 1024, "svd: sector larger than 1024 (Jacobi pair table)");
@@ -600,7 +970,6 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     const int64_t np = n + (n & 1);
     poff[q + 1] = poff[q] + np * (np + 1) / 2;
     eoff[q + 1] = eoff[q] + n;
-    maxpsize = std::max(maxpsize, np * (np + 1) / 2);
   }
   std::vector<int64_t> soff(nQ + 1, 0), selo(nQ + 1, 0);
   for (int q = 0; q < nQ; q++) {
@@ -616,6 +985,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   double* svals = dalloc<double>(soff[nQ]);
   int* ssec = dalloc<int>(soff[nQ]);
   int* order = dalloc<int>(soff[nQ]);
+  double* ssorted = dalloc<double>(soff[nQ]);
   int* newdim = dalloc<int>(nQ);
   double* tout = dalloc<double>(2);
   int64_t* d_moff = up(moff);
@@ -626,7 +996,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   int64_t* d_selo = up(selo);
   int* d_nq = up(nq);
   int* d_kq = up(kq);
-  for (void* p : std::initializer_list<void*>{mat, G, Vev, gs, ev, Esel, svals, ssec, order, newdim, tout,
+  for (void* p : std::initializer_list<void*>{mat, G, Vev, gs, ev, Esel, svals, ssec, order, ssorted, newdim, tout,
                                               d_moff, d_goff, d_poff, d_eoff, d_soff, d_selo, d_nq, d_kq})
     scratch.push_back(p);
   // ---- 2. gather
@@ -648,18 +1018,39 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     if (run_host_gemm(hg, mat, mat, G)) { release(); return 1; }
   }
   // ---- 4. eigensolver
-  const size_t lds_bytes = sizeof(double) * (size_t)std::min<int64_t>(maxpsize, kLdsDoubles);
-  TTG_HIP(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(sizeof(double) * kLdsDoubles)));
-  hipLaunchKernelGGL(k_jacobi, dim3(nQ), dim3(kJacThreads), lds_bytes, s, d_nq, d_goff, G, gs,
-                     d_poff, Vev, ev, d_eoff, 40);
-  TTG_HIP(hipGetLastError());
+  std::vector<int> big;
+  int64_t lds_need = 1;
+  int nsmall = 0;
+  for (int q = 0; q < nQ; q++) {
+    if (nq[q] > kJacBlockMin) { big.push_back(q); continue; }
+    lds_need = std::max(lds_need, jac_layout(nq[q]).total);
+    nsmall = std::max(nsmall, nq[q]);
+  }
+  if (nsmall > 0) {
+    const size_t lds_bytes = sizeof(double) * (size_t)std::min<int64_t>(lds_need, kLdsDoubles);
+    TTG_HIP(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * kLdsDoubles)));
+    // small sectors: fewer waves per workgroup (cheaper barriers per round)
+    const int jac_threads = nsmall <= 96 ? 256 : kJacThreads;
+    hipLaunchKernelGGL(k_jacobi, dim3(nQ), dim3(jac_threads), lds_bytes, s, d_nq, d_goff, G, gs,
+                       d_poff, Vev, ev, d_eoff, 40, kJacBlockMin);
+    TTG_HIP(hipGetLastError());
+  }
+  if (!big.empty() && block_jacobi(big, nq, goff, d_goff, d_eoff, G, Vev, ev, s, 40)) {
+    release();
+    return 1;
+  }
   // ---- 5. selection and truncation
   hipLaunchKernelGGL(k_svd_select, dim3(nQ), dim3(256), 0, s, d_nq, d_kq, d_eoff, ev, d_goff, Vev,
                      d_soff, svals, ssec, d_selo, Esel);
   TTG_HIP(hipMemsetAsync(newdim, 0, sizeof(int) * nQ, s));
-  hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(256), 0, s, (int)soff[nQ], svals, ssec, order,
-                     cutoff, K, newdim, tout);
+  {
+    const int T = (int)soff[nQ];
+    if (T > 0)
+      hipLaunchKernelGGL(k_svd_rank, dim3((T + 255) / 256), dim3(256), 0, s, T, svals, order, ssorted);
+    hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(64), 0, s, T, svals, ssec, order, ssorted,
+                       cutoff, K, newdim, tout);
+  }
   std::vector<int> nd(nQ);
   double th[2];
   TTG_HIP(hipMemcpyAsync(nd.data(), newdim, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));

@@ -281,6 +281,8 @@ double tth_psiphi(void* psi, void* phi) {
   TTH_GUARD(kNaN, return psiphi(*(qMPS<double>*)psi, *(qMPS<double>*)phi);)
 }
 
+void tth_set_dmrg_progress(int every) { set_dmrg_progress(every); }
+
 // dmrg.cpp:760-813.  stats: [t_davidson, t_svd, t_env, n_matvec, seconds]
 double tth_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int nc, const double* cut,
                 int nmr, const int* mr, double tol, double* halfsweep_E, double* halfsweep_s,

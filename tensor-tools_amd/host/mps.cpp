@@ -672,6 +672,9 @@ double psiphi(qMPS<double>& psi, qMPS<double>& phi) {
   return res;
 }
 
+static int g_progress_every = 0;
+void set_dmrg_progress(int every) { g_progress_every = every; }
+
 double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart, double tol,
             DmrgLog* log) {
@@ -691,6 +694,10 @@ double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vecto
     Energy = A.expec(psi.A[0]);
     if (log) log->initial_energy = Energy;
   }
+  if (g_progress_every > 0) {
+    fprintf(stderr, "[dmrg] canonical form and environments ready, E0=%.12f\n", Energy);
+    fflush(stderr);
+  }
   for (int sweep = 0; sweep < num_sweeps; sweep++) {
     const int lbd = sweep % (int)max_bd.size();
     const int lc = sweep % (int)cutoff.size();
@@ -706,6 +713,11 @@ double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vecto
         double te = now_s();
         updateEnv(psi, H, TR, TL, site, direction);
         ttg_sync();
+        if (g_progress_every > 0 && (i % g_progress_every == 0 || i == L - 2)) {
+          fprintf(stderr, "[dmrg] sweep %d half %d bond %d/%d E=%.12f kept=%u %.2fs\n", sweep,
+                  l % 2, i + 1, L - 1, sl.energy, sl.kept, now_s() - t0);
+          fflush(stderr);
+        }
         if (log) {
           log->bond_energy.push_back(sl.energy);
           log->trunc_err.push_back(sl.trunc_err);

@@ -276,6 +276,8 @@ struct DmrgLog {
   double t_dav = 0, t_svd = 0, t_env = 0;
   long n_matvec = 0;
 };
+// progress line on stderr every `every` bonds of a half-sweep (0: silent)
+void set_dmrg_progress(int every);
 double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart,
             double tol = 1e-12, DmrgLog* log = nullptr);

@@ -49,6 +49,7 @@ def _lib():
             "tth_qtt_length": ([_p], _i), "tth_qtt_center": ([_p], _i),
             "tth_qtt_set_center": ([_p, _i], None),
             "tth_psiHphi": ([_p, _p, _p], _d), "tth_psiphi": ([_p, _p], _d),
+            "tth_set_dmrg_progress": ([_i], None),
             "tth_mps_free": ([_p], None), "tth_mpo_free": ([_p], None),
             "tth_mps_position": ([_p, _i], _d), "tth_mps_normalize": ([_p], _i),
             "tth_qtt_bond_dims": ([_p, _ip], None),
@@ -284,6 +285,11 @@ def mpo_heisenberg_nn(N: int, mpo_id: int = 2) -> PMPO:
 def mps_product(state, mps_id: int = 1) -> PMPS:
     st = np.ascontiguousarray(np.array(state, dtype=np.int32))
     return PMPS(_lib().tth_mps_product(len(st), mps_id, st))
+
+
+def set_dmrg_progress(every: int):
+    """dmrg() prints a progress line to stderr every `every` bonds (0: silent)"""
+    _lib().tth_set_dmrg_progress(every)
 
 
 def psiHphi(psi: PMPS, H: PMPO, phi: PMPS) -> float:
