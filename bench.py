@@ -77,7 +77,8 @@ def sweep_leg(P, T, args):
         psi.set_center(-1)
         r = P.dmrg(psi, H, 1, [args.sweep_m], [1e-9], [2], tol=1e-12)
         sweep_s = float(np.sum(r["halfsweep_s"]))
-        res[name] = dict(sweep_s=round(sweep_s, 4), halfsweep_s=[round(v, 4) for v in r["halfsweep_s"]],
+        res[name] = dict(sweep_s=round(sweep_s, 4),
+                         halfsweep_s=[round(float(v), 4) for v in r["halfsweep_s"]],
                          total_s=round(r["seconds"], 4), E=r["E"], E_per_site=r["E"] / N,
                          n_matvec=r["n_matvec"], t_davidson=round(r["t_davidson"], 4),
                          t_svd=round(r["t_svd"], 4), t_env=round(r["t_env"], 4),
@@ -94,6 +95,30 @@ def sweep_leg(P, T, args):
     if args.cpu_seconds > 0 and args.sweep_m == m_w:
         out["cpu_baseline"] = cpu_bond_sample(args, N)
     return out
+
+
+def precision_check(P, T):
+    """Converged comparison of the precision forms: the N=20 open Heisenberg
+    chain, two-site DMRG from the Neel state, 10 sweeps, bond dimension
+    16 -> 64, cutoff 1e-10, Davidson max_restart 2, tol 1e-10; the final
+    energy per site of each form against the FP64 run and against exact
+    diagonalisation (tests/golden/ed_energies.json)."""
+    N = 20
+    ed = json.load(open(os.path.join(ROOT, "tests", "golden", "ed_energies.json")))["heisenberg_obc_N20"]
+    H = P.mpo_heisenberg_nn(N)
+    out = {}
+    for name, prec in (("fp64", T.FP64), ("fp32", T.FP32), ("bf16x3", T.BF16X3)):
+        P.set_precision(prec)
+        psi = P.mps_product([i % 2 for i in range(N)])
+        r = P.dmrg(psi, H, 10, [16, 32, 64, 64], [1e-10], [2], 1e-10)
+        out[name] = {"E_per_site": r["E"] / N, "dE_per_site_vs_ed": abs(r["E"] - ed) / N}
+    P.set_precision(T.FP64)
+    for v in out.values():
+        v["dE_per_site_vs_fp64"] = abs(v["E_per_site"] - out["fp64"]["E_per_site"])
+    log("[bench] precision check (N=20 chain, 10 sweeps): " + ", ".join(
+        f"{k} |dE|/site vs fp64 {v['dE_per_site_vs_fp64']:.2e} vs ED {v['dE_per_site_vs_ed']:.2e}"
+        for k, v in out.items()))
+    return {"system": "Heisenberg chain N=20 OBC, 10 sweeps, m 16->64, cutoff 1e-10", "forms": out}
 
 
 def cpu_bond_sample(args, N):
@@ -237,6 +262,8 @@ def main():
             f"GPU FP64 vs oracle rel err {cpu['fp64_gpu_vs_oracle_relerr']:.2e}")
 
     sweep = sweep_leg(P, T, args) if args.sweep_m > 0 else None
+    if sweep is not None:
+        sweep["precision_check"] = precision_check(P, T)
 
     lat, Nx, Ny, yper, J2, site, m, smax, sigma = WORKLOADS[args.workload]
     out = {

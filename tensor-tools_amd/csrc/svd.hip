@@ -199,8 +199,10 @@ __global__ void k_svd_gather(MetaView A, int nleft, const double* __restrict__ a
 // (i <= j at j(j+1)/2 + i).  Dynamic LDS holds, when they fit: the triangle,
 // the eigenvector matrix V and the table of 2x2 block pairs of a round;
 // otherwise V, then also the triangle, live in HBM (scratch / output).
-// A sweep converges when no rotation is significant: |a_pq| above 1e-15 of
-// sqrt(a_pp a_qq) and above 1e-14 of the largest diagonal.  Below that floor
+// A sweep converges when no rotation is significant: |a_pq| above 1e-12 of
+// sqrt(a_pp a_qq) (an off-diagonal d moves eigenvalues by ~d^2 / gap, so
+// the eigenvalues are exact to ~1e-24 relative) and above 1e-14 of the
+// largest diagonal.  Below that floor
 // a rotation moves no eigenvalue by more than 1e-14 of the largest (singular
 // values below 1e-7 of the largest, under the Gram matrix's own resolution of
 // ~sqrt(eps)), and the rounding noise in the null space of a rank-deficient
@@ -309,7 +311,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
         const double app = g[pk(a, a)], aqq = g[pk(b, b)], apq = g[pk(a, b)];
         double c = 1.0, s = 0.0;
         if (apq != 0.0) {
-          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > floor_abs) any_rot = 1;
+          if (fabs(apq) > 1e-12 * sqrt(fabs(app * aqq)) && fabs(apq) > floor_abs) any_rot = 1;
           const double theta = (aqq - app) / (2.0 * apq);
           const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
           c = 1.0 / sqrt(t * t + 1.0);
@@ -375,7 +377,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
 // so a round moves G and V through memory once instead of once per scalar
 // rotation.  A sweep (nbk - 1 rounds) leaves a sector active while any
 // subproblem held an off-diagonal element above the scalar kernel's
-// threshold (1e-15 sqrt(g_ii g_jj) and 1e-14 of the largest diagonal).
+// threshold (1e-12 sqrt(g_ii g_jj) and 1e-14 of the largest diagonal).
 constexpr int kJacBlockMin = 64;
 constexpr int kBJ = 32;                // block width; subproblems are 2 kBJ
 constexpr int kSub = 2 * kBJ;
@@ -447,7 +449,7 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
     if (i == j) continue;
     const double v = Ss[i * kSubLd + j];
     const double dd = sqrt(fabs(Ss[i * kSubLd + i] * Ss[j * kSubLd + j]));
-    any |= fabs(v) > 1e-15 * dd && fabs(v) > 1e-14 * S.dmax;
+    any |= fabs(v) > 1e-12 * dd && fabs(v) > 1e-14 * S.dmax;
   }
   if (any) sig = 1;
   __syncthreads();
