@@ -11,6 +11,10 @@
 // of the same sector tuple in x, so y and x share one block list and the
 // Davidson vector ops stay plain packed-array BLAS-1.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <tuple>
 
 #include "ttg_device.h"
 #include "ttg_internal.h"
@@ -97,6 +101,56 @@ static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
   return 0;
 }
 
+// Measured tile-shape choice for an MFMA stage of an operator: each MFMA
+// tile shape's work list is built and its launch timed (one untimed, two
+// timed, HIP events on the library stream) on the stage's own operands; the
+// fastest shape is kept.  The choice is cached by a coarse signature of the
+// stage (precision, stage, rounded log2 flops and tasks, operand layout), so
+// the operators of neighbouring DMRG bonds reuse it without timing.  Stages
+// the planner gave the vector-ALU kernel, stages below kTuneMinFlops (their
+// launches are short whatever the shape) and pinned shapes (TTG_TILE_CFG)
+// keep the analytic choice.
+struct TuneKey {
+  int prec, stage, lfl, lnt, layout;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(prec, stage, lfl, lnt, layout) < std::tie(o.prec, o.stage, o.lfl, o.lnt, o.layout);
+  }
+};
+static std::map<TuneKey, int> g_tuned;
+constexpr double kTuneMinFlops = 2e8;
+
+static int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, bool out_f64,
+                      hipStream_t s) {
+  if (p->cfg == TILE_SKINNY || p->ntasks == 0 || p->flops < kTuneMinFlops) return 0;
+  const char* e = getenv("TTG_TILE_CFG");
+  if (e && *e) return 0;
+  const TuneKey key{p->prec, stage, (int)std::lround(std::log2(std::max(p->flops, 1.0))),
+                    (int)std::lround(std::log2((double)std::max<int64_t>(p->ntasks, 1))),
+                    (p->a_mcontig ? 1 : 0) | (p->b_kcontig ? 2 : 0)};
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) return it->second == p->cfg ? 0 : rebuild_tiles(*p, it->second);
+  hipEvent_t e0, e1;
+  TTG_HIP(hipEventCreate(&e0));
+  TTG_HIP(hipEventCreate(&e1));
+  int best = p->cfg;
+  float best_ms = 1e30f;
+  for (int c = 0; c < TILE_NCFG; c++) {
+    if (rebuild_tiles(*p, c)) { hipEventDestroy(e0); hipEventDestroy(e1); return 1; }
+    TTG_TRY(gemm_launch(*p, A, B, C, out_f64, s));
+    TTG_HIP(hipEventRecord(e0, s));
+    for (int rep = 0; rep < 2; rep++) TTG_TRY(gemm_launch(*p, A, B, C, out_f64, s));
+    TTG_HIP(hipEventRecord(e1, s));
+    TTG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    TTG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  g_tuned[key] = best;
+  return rebuild_tiles(*p, best);
+}
+
 static int record(ttg_heff h, hipStream_t s) {
   const bool red = h->prec != TTG_FP64;
   const size_t nx = h->xm->total;
@@ -166,8 +220,19 @@ int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_lik
     h->x32 = dalloc<float>(h->xm->total);
     if (to_f32(L, &h->L32) || to_f32(W12, &h->W32) || to_f32(R, &h->R32)) { delete h; return 1; }
   }
-  // capture the apply sequence once
   hipStream_t s = stream();
+  {   // measured tile shapes of the MFMA stages (before the graph is captured)
+    const void* a1 = red ? (const void*)h->L32 : (const void*)L->d_data;
+    const void* b1 = red ? (const void*)h->x32 : (const void*)h->xbuf;
+    const void* b2 = red ? (const void*)h->W32 : (const void*)W12->d_data;
+    const void* b3 = red ? (const void*)h->R32 : (const void*)R->d_data;
+    if (tune_stage(h->p[0], 0, a1, b1, h->T1, !red, s) || tune_stage(h->p[1], 1, h->T1, b2, h->T2, !red, s) ||
+        tune_stage(h->p[2], 2, h->T2, b3, h->ybuf, true, s)) {
+      delete h;
+      return 1;
+    }
+  }
+  // capture the apply sequence once
   TTG_HIP(hipStreamSynchronize(s));
   TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rr = record(h, s);

@@ -584,6 +584,22 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
   return 0;
 }
 
+// replace a plan's work list by one for tile shape cfg (an MFMA shape)
+int rebuild_tiles(ttg_plan_s& p, int cfg) {
+  TTG_REQUIRE(cfg >= 0 && cfg < TILE_NCFG, "plan: bad tile shape");
+  std::vector<CBlock> hcb(p.nblocksC);
+  hipStream_t s = stream();
+  if (p.nblocksC)
+    TTG_HIP(hipMemcpyAsync(hcb.data(), p.d_cblk, sizeof(CBlock) * p.nblocksC, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  dfree(p.d_tiles); dfree(p.d_groups); dfree(p.d_part);
+  p.d_tiles = nullptr; p.d_groups = nullptr; p.d_part = nullptr;
+  p.ngroups = 0;
+  p.nslots = 0;
+  p.cfg = cfg;
+  return build_tiles(p, hcb);
+}
+
 int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
                int force_cfg, ttg_plan* out) {
   TTG_TRY(ensure_init());

@@ -480,27 +480,48 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
       cs_s[tid] = sn;
     }
     __syncthreads();
-    // rows a, b of S
-    for (int e = tid; e < kBJ * kSub; e += kSolveThreads) {
-      const int p = e / kSub, j = e % kSub;
-      const int a = pa[p], b = pb[p];
-      const double c = cs_c[p], sn = cs_s[p];
-      const double xa = Ss[a * kSubLd + j], xb = Ss[b * kSubLd + j];
-      Ss[a * kSubLd + j] = c * xa - sn * xb;
-      Ss[b * kSubLd + j] = sn * xa + c * xb;
+    // thread: element j (or i) = tid % 64 of the rotations p = tid / 64 + 4 u;
+    // the rotations are disjoint, so all loads go out before any store
+    constexpr int NU = kBJ * kSub / kSolveThreads;
+    const int el = tid % kSub, p0 = tid / kSub;
+    {   // rows a, b of S
+      double xa[NU], xb[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int p = p0 + 4 * u;
+        xa[u] = Ss[pa[p] * kSubLd + el];
+        xb[u] = Ss[pb[p] * kSubLd + el];
+      }
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int p = p0 + 4 * u;
+        const double c = cs_c[p], sn = cs_s[p];
+        Ss[pa[p] * kSubLd + el] = c * xa[u] - sn * xb[u];
+        Ss[pb[p] * kSubLd + el] = sn * xa[u] + c * xb[u];
+      }
     }
     __syncthreads();
-    // columns a, b of S and Q
-    for (int e = tid; e < kBJ * kSub; e += kSolveThreads) {
-      const int p = e / kSub, i = e % kSub;
-      const int a = pa[p], b = pb[p];
-      const double c = cs_c[p], sn = cs_s[p];
-      const double xa = Ss[i * kSubLd + a], xb = Ss[i * kSubLd + b];
-      Ss[i * kSubLd + a] = c * xa - sn * xb;
-      Ss[i * kSubLd + b] = sn * xa + c * xb;
-      const double qa = Qs[i * kSubLd + a], qb = Qs[i * kSubLd + b];
-      Qs[i * kSubLd + a] = c * qa - sn * qb;
-      Qs[i * kSubLd + b] = sn * qa + c * qb;
+    {   // columns a, b of S and Q
+      double xa[NU], xb[NU], qa[NU], qb[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int p = p0 + 4 * u;
+        const int a = pa[p], b = pb[p];
+        xa[u] = Ss[el * kSubLd + a];
+        xb[u] = Ss[el * kSubLd + b];
+        qa[u] = Qs[el * kSubLd + a];
+        qb[u] = Qs[el * kSubLd + b];
+      }
+#pragma unroll
+      for (int u = 0; u < NU; u++) {
+        const int p = p0 + 4 * u;
+        const int a = pa[p], b = pb[p];
+        const double c = cs_c[p], sn = cs_s[p];
+        Ss[el * kSubLd + a] = c * xa[u] - sn * xb[u];
+        Ss[el * kSubLd + b] = sn * xa[u] + c * xb[u];
+        Qs[el * kSubLd + a] = c * qa[u] - sn * qb[u];
+        Qs[el * kSubLd + b] = sn * qa[u] + c * qb[u];
+      }
     }
     __syncthreads();
   }

@@ -116,4 +116,6 @@ int gemm_occupancy(int prec, int cfg, bool a_mcontig, bool b_kcontig);
 // work list of a plan from its output blocks (host copy): tiles of the
 // chosen shape, split along K when the tile count alone cannot fill the GPU
 int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb);
+// the same plan's work list rebuilt for another MFMA tile shape
+int rebuild_tiles(ttg_plan_s& p, int cfg);
 }  // namespace ttg
