@@ -371,14 +371,17 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
 // Gram matrix (N = n rounded up to 64, zero padding never rotates) is cut into
 // 32-wide blocks; a round pairs the blocks (circle method) and
 //   k_bjac_solve  diagonalises every 64 x 64 subproblem G[I+J, I+J] in LDS
-//                 (one inner cyclic sweep, one wave per pair) into Q_IJ, and
-//   k_bjac_apply  updates G <- P^T G P and V <- V P, P = diag(Q_IJ), as
-//                 64 x 64 tile products,
+//                 (one inner cyclic sweep, 256 threads per pair) into Q_IJ,
+//                 its eigenpairs sorted in descending order, and
+//   k_bjac_apply  updates G <- P^T G P (upper tiles, mirrored: G stays
+//                 exactly symmetric) and V <- V P, P = diag(Q_IJ), as 64 x 64
+//                 tile products,
 // so a round moves G and V through memory once instead of once per scalar
 // rotation.  A sweep (nbk - 1 rounds) leaves a sector active while any
-// subproblem held an off-diagonal element above the scalar kernel's
-// threshold (1e-12 sqrt(g_ii g_jj) and 1e-14 of the largest diagonal).
+// subproblem held an off-diagonal element above 1e-12 sqrt(g_ii g_jj) and
+// above the sector's floor, 1e-14 of the largest diagonal.
 constexpr int kJacBlockMin = 64;
+constexpr double kBJRel = 1e-12, kBJFloor = 1e-14;
 constexpr int kBJ = 32;                // block width; subproblems are 2 kBJ
 constexpr int kSub = 2 * kBJ;
 constexpr int kSubLd = kSub + 1;       // LDS row pitch (bank spread)
@@ -386,7 +389,8 @@ constexpr int kSubLd = kSub + 1;       // LDS row pitch (bank spread)
 struct BJSector {
   int n, N, nbk, q;                    // size, padded size, blocks, sector id
   int64_t gp, qp;                      // offsets of padded G / V and of the pair Q's
-  double dmax;
+  double dmax;                         // largest diagonal of the Gram matrix
+  double floor;                        // off-diagonal size below which nothing rotates
 };
 
 // index (0..63) of subproblem entry t for the pair k of round r
@@ -421,6 +425,7 @@ __global__ void k_bjac_init(const BJSector* sec, int nsec, const int64_t* goff, 
 // sweep; each round's 32 disjoint rotations are spread over the 256 threads
 // as (rotation, element) pairs
 constexpr int kSolveThreads = 256;
+constexpr int kInnerSweeps = 1;   // inner cyclic sweeps per subproblem (more: fewer outer sweeps, not faster)
 __global__ void __launch_bounds__(kSolveThreads)
 k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const int* active, int* flag,
              const double* Gp, double* Qp) {
@@ -442,6 +447,15 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
     Qs[i * kSubLd + j] = i == j ? 1.0 : 0.0;
   }
   __syncthreads();
+  for (int e = tid; e < kSub * kSub; e += kSolveThreads) {   // exact symmetry
+    const int i = e % kSub, j = e / kSub;
+    if (i < j) {
+      const double v = 0.5 * (Ss[i * kSubLd + j] + Ss[j * kSubLd + i]);
+      Ss[i * kSubLd + j] = v;
+      Ss[j * kSubLd + i] = v;
+    }
+  }
+  __syncthreads();
   // significance test of this subproblem (before rotating)
   bool any = false;
   for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
@@ -449,7 +463,7 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
     if (i == j) continue;
     const double v = Ss[i * kSubLd + j];
     const double dd = sqrt(fabs(Ss[i * kSubLd + i] * Ss[j * kSubLd + j]));
-    any |= fabs(v) > 1e-12 * dd && fabs(v) > 1e-14 * S.dmax;
+    any |= fabs(v) > kBJRel * dd && fabs(v) > S.floor;
   }
   if (any) sig = 1;
   __syncthreads();
@@ -460,6 +474,23 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
   }
   if (tid == 0) atomicOr(&flag[li], 1);
   const int m1 = kSub - 1;
+  for (int isw = 0; isw < kInnerSweeps; isw++) {
+  if (isw > 0) {   // stop the inner sweeps once the subproblem is diagonal
+    __syncthreads();
+    if (tid == 0) sig = 0;
+    __syncthreads();
+    bool more = false;
+    for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
+      const int i = e % kSub, j = e / kSub;
+      if (i == j) continue;
+      const double v = Ss[i * kSubLd + j];
+      const double dd = sqrt(fabs(Ss[i * kSubLd + i] * Ss[j * kSubLd + j]));
+      more |= fabs(v) > kBJRel * dd && fabs(v) > S.floor;
+    }
+    if (more) sig = 1;
+    __syncthreads();
+    if (!sig) break;
+  }
   for (int round = 0; round < m1; round++) {
     if (tid < kBJ) {
       int a, b;
@@ -525,13 +556,29 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
     }
     __syncthreads();
   }
+  }
+  // eigenpairs in descending order (the larger values to the pair's lower
+  // block): a fixed order keeps values from swapping between blocks from
+  // round to round, which otherwise holds the outer iteration at a linear rate
+  __shared__ int perm[kSub];
+  if (tid < kSub) {
+    const double d = Ss[tid * kSubLd + tid];
+    int rk = 0;
+    for (int u = 0; u < kSub; u++) {
+      const double du = Ss[u * kSubLd + u];
+      rk += (du > d) || (du == d && u < tid);
+    }
+    perm[tid] = rk;
+  }
+  __syncthreads();
   for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
     const int i = e % kSub, j = e / kSub;
-    Q[e] = Qs[i * kSubLd + j];
+    Q[i + kSub * perm[j]] = Qs[i * kSubLd + j];
   }
 }
 
-// tile products of a round: kind 0: G[Ia, Ib] <- Qa^T G[Ia, Ib] Qb;
+// tile products of a round: kind 0 (a <= b): G[Ia, Ib] <- Qa^T G[Ia, Ib] Qb,
+// and its transpose into G[Ib, Ia];
 // kind 1: V[rows r0..r0+63, Ib] <- V[.., Ib] Qb
 __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const int4* items, int nitems,
                                                     int r, const int* active, double* Gp, double* Vp,
@@ -590,6 +637,7 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
       const int gi = kind == 0 ? bj_index(S.nbk, r, a, i0 + x) : a * kSub + i0 + x;
       const int gj = bj_index(S.nbk, r, b, j0 + y);
       M[S.gp + gi + (int64_t)S.N * gj] = u[x][y];
+      if (kind == 0 && a != b) M[S.gp + gj + (int64_t)S.N * gi] = u[x][y];   // G stays symmetric
     }
 }
 
@@ -624,13 +672,13 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
     const int q = big[li], n = nq[q];
     const int N = (n + kSub - 1) / kSub * kSub;
     const int nbk = N / kBJ, npair = nbk / 2;
-    sec[li] = BJSector{n, N, nbk, q, gp, qp, 0.0};
+    sec[li] = BJSector{n, N, nbk, q, gp, qp, 0.0, 0.0};
     gp += (int64_t)N * N;
     qp += (int64_t)npair * kSub * kSub;
     maxr = std::max(maxr, nbk - 1);
     for (int k = 0; k < npair; k++) pitems.push_back(int2{li, k});
     for (int a = 0; a < npair; a++)
-      for (int b = 0; b < npair; b++) titems.push_back(int4{li, a, b, 0});
+      for (int b = a; b < npair; b++) titems.push_back(int4{li, a, b, 0});   // a <= b, mirrored
     for (int rc = 0; rc < N / kSub; rc++)
       for (int b = 0; b < npair; b++) titems.push_back(int4{li, rc, b, 1});
   }
@@ -643,6 +691,7 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
     TTG_HIP(hipStreamSynchronize(s));
     for (double v : d) dmax[li] = std::max(dmax[li], std::fabs(v));
     sec[li].dmax = dmax[li];
+    sec[li].floor = kBJFloor * dmax[li];
   }
   BJSector* d_sec = dalloc<BJSector>(ns);
   int2* d_pit = dalloc<int2>(pitems.size());
