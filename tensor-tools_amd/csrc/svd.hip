@@ -371,39 +371,45 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
 // Gram matrix (N = n rounded up to 64, zero padding never rotates) is cut into
 // 32-wide blocks; a round pairs the blocks (circle method) and
 //   k_bjac_solve  diagonalises every 64 x 64 subproblem G[I+J, I+J] in LDS
-//                 (one inner cyclic sweep, 256 threads per pair) into Q_IJ,
-//                 its eigenpairs sorted in descending order, and
+//                 (one inner cyclic sweep) into Q_IJ, its eigenpairs sorted
+//                 in descending order, and
 //   k_bjac_apply  updates G <- P^T G P (upper tiles, mirrored: G stays
 //                 exactly symmetric) and V <- V P, P = diag(Q_IJ), as 64 x 64
-//                 tile products,
+//                 tile products on the FP64 matrix cores,
 // so a round moves G and V through memory once instead of once per scalar
 // rotation.  A sweep (nbk - 1 rounds) leaves a sector active while any
 // subproblem held an off-diagonal element above 1e-12 sqrt(g_ii g_jj) and
-// above the sector's floor, 1e-14 of the largest diagonal.
+// above the sector's floor, 1e-14 of the largest diagonal.  A subproblem
+// with nothing to rotate leaves Q_IJ = I and marks its pair unrotated; the
+// tile products of a round skip the tiles whose pairs were both unrotated.
 constexpr int kJacBlockMin = 64;
 constexpr double kBJRel = 1e-12, kBJFloor = 1e-14;
 constexpr int kBJ = 32;                // block width; subproblems are 2 kBJ
 constexpr int kSub = 2 * kBJ;
 constexpr int kSubLd = kSub + 1;       // LDS row pitch (bank spread)
 
+typedef double bj_v4d __attribute__((ext_vector_type(4)));
+
 struct BJSector {
   int n, N, nbk, q;                    // size, padded size, blocks, sector id
   int64_t gp, qp;                      // offsets of padded G / V and of the pair Q's
   double dmax;                         // largest diagonal of the Gram matrix
   double floor;                        // off-diagonal size below which nothing rotates
+  int pb, pad;                         // index of the sector's first pair in the pair list
 };
 
-// index (0..63) of subproblem entry t for the pair k of round r
-__device__ inline int bj_pair_block(int nbk, int r, int k, int half) {
-  const int m1 = nbk - 1;
-  int a, b;
+// circle method over m = 2 npair indices: pair k of round r (a < b)
+__device__ inline void circle_pair(int m, int r, int k, int& a, int& b) {
+  const int m1 = m - 1;
   if (k == 0) { a = r % m1; b = m1; }
   else { a = (r + k) % m1; b = (r - k + m1) % m1; }
-  if (a > b) { int t = a; a = b; b = t; }
-  return half ? b : a;
+  if (a > b) { const int t = a; a = b; b = t; }
 }
+// global index (0..N-1) of subproblem entry t (0..63) for the pair k of round r
 __device__ inline int bj_index(int nbk, int r, int k, int t) {
-  return bj_pair_block(nbk, r, k, t >= kBJ) * kBJ + (t & (kBJ - 1));
+  int a, b;
+  circle_pair(nbk, r, k, a, b);
+  return (t >= kBJ ? b : a) * kBJ + (t & (kBJ - 1));
 }
 
 __global__ void k_bjac_init(const BJSector* sec, int nsec, const int64_t* goff, const double* G,
@@ -421,224 +427,281 @@ __global__ void k_bjac_init(const BJSector* sec, int nsec, const int64_t* goff, 
   }
 }
 
+// Jacobi rotation zeroing a_pq of [[app, apq], [apq, aqq]]: rows/columns
+// (p, q) <- (c x_p - s x_q, s x_p + c x_q), t = s/c the smaller root of
+// t^2 + 2 theta t - 1 = 0, theta = (aqq - app) / (2 apq).  Reciprocal and
+// reciprocal square root from the hardware estimates with two Newton steps
+// each (full double precision; c^2 + s^2 = 1 to rounding).
+__device__ inline double rcp_nr(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ inline double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+__device__ inline void jac_rotation(double app, double aqq, double apq, double& c, double& s) {
+  c = 1.0;
+  s = 0.0;
+  if (!(fabs(apq) > 1e-290)) return;   // nothing to rotate (and no overflow below)
+  const double theta = (aqq - app) * (0.5 * rcp_nr(apq));
+  const double at = fabs(theta);
+  double t;
+  if (at < 1e150) {
+    const double h = fma(theta, theta, 1.0);
+    t = rcp_nr(at + h * rsq_nr(h));
+  } else {
+    t = 0.5 * rcp_nr(at);
+  }
+  if (theta < 0) t = -t;
+  c = rsq_nr(fma(t, t, 1.0));
+  s = t * c;
+}
+
 // one workgroup per (sector, pair): the 64 x 64 subproblem, one cyclic
-// sweep; each round's 32 disjoint rotations are spread over the 256 threads
-// as (rotation, element) pairs
+// sweep of 63 rounds of 32 disjoint rotations (circle method), in slot
+// order: slot k holds the pair of round r at positions (2k, 2k+1), so every
+// rotation acts on adjacent rows and columns.  Between rounds the indices
+// move by the fixed permutation slot_next (period 63); a round reads S and
+// Q^T from one buffer and writes the rotated values to their next-round
+// positions in the other.  A round:
+//   1. 32 threads: rotation of each slot from its 2x2 diagonal block;
+//   2. all threads: S <- M S M^T (thread t: rows 2k, 2k+1, k = t/8, and the
+//      column pairs k' = t%8 + 8u) and Q^T <- M Q^T (the same rows).
+constexpr int kSP = 72;                    // LDS row pitch (doubles) of S and Q^T
+constexpr int kSlotMat = kSub * kSP;
+constexpr size_t kSolveLds = (4 * kSlotMat + 2 * kBJ) * sizeof(double) + (kSub + 1) * sizeof(int);
+__device__ inline int slot_of(int x) { return x < kBJ ? 2 * x : 2 * (kSub - 1 - x) + 1; }
+__device__ inline int slot_next(int p) {
+  const int k = p >> 1;
+  if (!(p & 1)) return k ? 2 * (k - 1) : 3;
+  if (k == 0) return 1;
+  return k < kBJ - 1 ? 2 * k + 3 : 2 * kBJ - 2;
+}
+
 constexpr int kSolveThreads = 256;
-constexpr int kInnerSweeps = 1;   // inner cyclic sweeps per subproblem (more: fewer outer sweeps, not faster)
 __global__ void __launch_bounds__(kSolveThreads)
 k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const int* active, int* flag,
-             const double* Gp, double* Qp) {
-  __shared__ double Ss[kSub * kSubLd];
-  __shared__ double Qs[kSub * kSubLd];
-  __shared__ double cs_c[kBJ], cs_s[kBJ];
-  __shared__ int pa[kBJ], pb[kBJ];
-  __shared__ int sig;
+             int* rotated, const double* Gp, double* Qp) {
+  extern __shared__ __attribute__((aligned(16))) double bj_lds[];
+  double* cs = bj_lds + 4 * kSlotMat;          // (c_k, s_k) at 2k, 2k+1
+  int* perm = (int*)(cs + 2 * kBJ);
+  int* sig = perm + kSub;
   const int2 it = items[blockIdx.x];
   const int li = it.x, k = it.y;
   const BJSector S = sec[li];
   if (!active[li] || r >= S.nbk - 1) return;
   const int tid = threadIdx.x;
-  if (tid == 0) sig = 0;
-  for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
-    const int i = e % kSub, j = e / kSub;
-    const int gi = bj_index(S.nbk, r, k, i), gj = bj_index(S.nbk, r, k, j);
-    Ss[i * kSubLd + j] = Gp[S.gp + gi + (int64_t)S.N * gj];
-    Qs[i * kSubLd + j] = i == j ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  for (int e = tid; e < kSub * kSub; e += kSolveThreads) {   // exact symmetry
-    const int i = e % kSub, j = e / kSub;
-    if (i < j) {
-      const double v = 0.5 * (Ss[i * kSubLd + j] + Ss[j * kSubLd + i]);
-      Ss[i * kSubLd + j] = v;
-      Ss[j * kSubLd + i] = v;
+  if (tid == 0) *sig = 0;
+  {
+    double* S0 = bj_lds;
+    double* Q0 = bj_lds + 2 * kSlotMat;
+    for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
+      const int i = e % kSub, j = e / kSub;   // G is exactly symmetric
+      const int gi = bj_index(S.nbk, r, k, i), gj = bj_index(S.nbk, r, k, j);
+      S0[slot_of(i) * kSP + slot_of(j)] = Gp[S.gp + gi + (int64_t)S.N * gj];
+      Q0[slot_of(j) * kSP + i] = i == j ? 1.0 : 0.0;
     }
   }
   __syncthreads();
-  // significance test of this subproblem (before rotating)
-  bool any = false;
-  for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
-    const int i = e % kSub, j = e / kSub;
-    if (i == j) continue;
-    const double v = Ss[i * kSubLd + j];
-    const double dd = sqrt(fabs(Ss[i * kSubLd + i] * Ss[j * kSubLd + j]));
-    any |= fabs(v) > kBJRel * dd && fabs(v) > S.floor;
+  {
+    const double* S0 = bj_lds;
+    bool any = false;   // significance test of this subproblem (before rotating)
+    for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
+      const int p = e % kSub, q = e / kSub;
+      if (p == q) continue;
+      const double v = S0[p * kSP + q];
+      const double dd = sqrt(fabs(S0[p * kSP + p] * S0[q * kSP + q]));
+      any |= fabs(v) > kBJRel * dd && fabs(v) > S.floor;
+    }
+    if (any) *sig = 1;
   }
-  if (any) sig = 1;
   __syncthreads();
-  double* Q = Qp + S.qp + (int64_t)k * kSub * kSub;   // column-major 64 x 64
-  if (!sig) {   // already diagonal: Q = I
-    for (int e = tid; e < kSub * kSub; e += kSolveThreads) Q[e] = (e % kSub == e / kSub) ? 1.0 : 0.0;
+  double* Qg = Qp + S.qp + (int64_t)k * kSub * kSub;   // column-major 64 x 64
+  if (!*sig) {   // already diagonal: Q = I
+    for (int e = tid; e < kSub * kSub; e += kSolveThreads) Qg[e] = (e % kSub == e / kSub) ? 1.0 : 0.0;
+    if (tid == 0) rotated[blockIdx.x] = 0;
     return;
   }
-  if (tid == 0) atomicOr(&flag[li], 1);
-  const int m1 = kSub - 1;
-  for (int isw = 0; isw < kInnerSweeps; isw++) {
-  if (isw > 0) {   // stop the inner sweeps once the subproblem is diagonal
-    __syncthreads();
-    if (tid == 0) sig = 0;
-    __syncthreads();
-    bool more = false;
-    for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
-      const int i = e % kSub, j = e / kSub;
-      if (i == j) continue;
-      const double v = Ss[i * kSubLd + j];
-      const double dd = sqrt(fabs(Ss[i * kSubLd + i] * Ss[j * kSubLd + j]));
-      more |= fabs(v) > kBJRel * dd && fabs(v) > S.floor;
-    }
-    if (more) sig = 1;
-    __syncthreads();
-    if (!sig) break;
+  if (tid == 0) {
+    atomicOr(&flag[li], 1);
+    rotated[blockIdx.x] = 1;
   }
-  for (int round = 0; round < m1; round++) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const int kk = tid >> 3, c8 = tid & 7;
+  const int r0 = 2 * kk, r1 = 2 * kk + 1;
+  const int nr0 = slot_next(r0) * kSP, nr1 = slot_next(r1) * kSP;
+  int ncol[4][2];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    ncol[u][0] = slot_next(2 * c8 + 16 * u);
+    ncol[u][1] = slot_next(2 * c8 + 16 * u + 1);
+  }
+  for (int round = 0; round < kSub - 1; round++) {
+    const double* Sc = bj_lds + (round & 1) * kSlotMat;
+    double* Sn = bj_lds + ((round & 1) ^ 1) * kSlotMat;
+    const double* Qc = bj_lds + (2 + (round & 1)) * kSlotMat;
+    double* Qn = bj_lds + (2 + ((round & 1) ^ 1)) * kSlotMat;
     if (tid < kBJ) {
-      int a, b;
-      if (tid == 0) { a = round; b = m1; }
-      else { a = (round + tid) % m1; b = (round - tid + m1) % m1; }
-      if (a > b) { int t = a; a = b; b = t; }
-      pa[tid] = a;
-      pb[tid] = b;
-      const double app = Ss[a * kSubLd + a], aqq = Ss[b * kSubLd + b], apq = Ss[a * kSubLd + b];
-      double c = 1.0, sn = 0.0;
-      if (apq != 0.0) {
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        c = 1.0 / sqrt(tt * tt + 1.0);
-        sn = tt * c;
-      }
-      cs_c[tid] = c;
-      cs_s[tid] = sn;
+      const int p = 2 * tid;
+      double c, s;
+      jac_rotation(Sc[p * kSP + p], Sc[(p + 1) * kSP + p + 1], Sc[p * kSP + p + 1], c, s);
+      *(d2*)&cs[p] = d2{c, s};
     }
     __syncthreads();
-    // thread: element j (or i) = tid % 64 of the rotations p = tid / 64 + 4 u;
-    // the rotations are disjoint, so all loads go out before any store
-    constexpr int NU = kBJ * kSub / kSolveThreads;
-    const int el = tid % kSub, p0 = tid / kSub;
-    {   // rows a, b of S
-      double xa[NU], xb[NU];
+    d2 x0[4], x1[4], q0[4], q1[4], cp[4];
 #pragma unroll
-      for (int u = 0; u < NU; u++) {
-        const int p = p0 + 4 * u;
-        xa[u] = Ss[pa[p] * kSubLd + el];
-        xb[u] = Ss[pb[p] * kSubLd + el];
-      }
-#pragma unroll
-      for (int u = 0; u < NU; u++) {
-        const int p = p0 + 4 * u;
-        const double c = cs_c[p], sn = cs_s[p];
-        Ss[pa[p] * kSubLd + el] = c * xa[u] - sn * xb[u];
-        Ss[pb[p] * kSubLd + el] = sn * xa[u] + c * xb[u];
-      }
+    for (int u = 0; u < 4; u++) {
+      x0[u] = *(const d2*)&Sc[r0 * kSP + 2 * c8 + 16 * u];
+      x1[u] = *(const d2*)&Sc[r1 * kSP + 2 * c8 + 16 * u];
+      q0[u] = *(const d2*)&Qc[r0 * kSP + 2 * c8 + 16 * u];
+      q1[u] = *(const d2*)&Qc[r1 * kSP + 2 * c8 + 16 * u];
+      cp[u] = *(const d2*)&cs[2 * c8 + 16 * u];
     }
-    __syncthreads();
-    {   // columns a, b of S and Q
-      double xa[NU], xb[NU], qa[NU], qb[NU];
+    const d2 ck = *(const d2*)&cs[2 * kk];
+    const double c1 = ck.x, s1 = ck.y;
 #pragma unroll
-      for (int u = 0; u < NU; u++) {
-        const int p = p0 + 4 * u;
-        const int a = pa[p], b = pb[p];
-        xa[u] = Ss[el * kSubLd + a];
-        xb[u] = Ss[el * kSubLd + b];
-        qa[u] = Qs[el * kSubLd + a];
-        qb[u] = Qs[el * kSubLd + b];
-      }
-#pragma unroll
-      for (int u = 0; u < NU; u++) {
-        const int p = p0 + 4 * u;
-        const int a = pa[p], b = pb[p];
-        const double c = cs_c[p], sn = cs_s[p];
-        Ss[el * kSubLd + a] = c * xa[u] - sn * xb[u];
-        Ss[el * kSubLd + b] = sn * xa[u] + c * xb[u];
-        Qs[el * kSubLd + a] = c * qa[u] - sn * qb[u];
-        Qs[el * kSubLd + b] = sn * qa[u] + c * qb[u];
-      }
+    for (int u = 0; u < 4; u++) {
+      // rows 2k, 2k+1 of the 2x2 block (k, k' = c8 + 8u), then its columns
+      const double y00 = c1 * x0[u].x - s1 * x1[u].x, y01 = c1 * x0[u].y - s1 * x1[u].y;
+      const double y10 = s1 * x0[u].x + c1 * x1[u].x, y11 = s1 * x0[u].y + c1 * x1[u].y;
+      const double c2 = cp[u].x, s2 = cp[u].y;
+      double z00 = c2 * y00 - s2 * y01, z01 = s2 * y00 + c2 * y01;
+      double z10 = c2 * y10 - s2 * y11, z11 = s2 * y10 + c2 * y11;
+      if (c8 + 8 * u == kk) { z01 = 0.0; z10 = 0.0; }   // the pair's own element is rotated away
+      Sn[nr0 + ncol[u][0]] = z00;
+      Sn[nr0 + ncol[u][1]] = z01;
+      Sn[nr1 + ncol[u][0]] = z10;
+      Sn[nr1 + ncol[u][1]] = z11;
+      *(d2*)&Qn[nr0 + 2 * c8 + 16 * u] = c1 * q0[u] - s1 * q1[u];
+      *(d2*)&Qn[nr1 + 2 * c8 + 16 * u] = s1 * q0[u] + c1 * q1[u];
     }
     __syncthreads();
   }
-  }
-  // eigenpairs in descending order (the larger values to the pair's lower
+  // 63 rounds return every index to its first slot; the result is in buffer 1.
+  // Eigenpairs in descending order (the larger values to the pair's lower
   // block): a fixed order keeps values from swapping between blocks from
   // round to round, which otherwise holds the outer iteration at a linear rate
-  __shared__ int perm[kSub];
+  const double* Sf = bj_lds + kSlotMat;
+  const double* Qf = bj_lds + 3 * kSlotMat;
   if (tid < kSub) {
-    const double d = Ss[tid * kSubLd + tid];
+    const double d = Sf[tid * kSP + tid];
     int rk = 0;
     for (int u = 0; u < kSub; u++) {
-      const double du = Ss[u * kSubLd + u];
+      const double du = Sf[u * kSP + u];
       rk += (du > d) || (du == d && u < tid);
     }
     perm[tid] = rk;
   }
   __syncthreads();
   for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
-    const int i = e % kSub, j = e / kSub;
-    Q[i + kSub * perm[j]] = Qs[i * kSubLd + j];
+    const int i = e % kSub, p = e / kSub;
+    Qg[i + kSub * perm[p]] = Qf[p * kSP + i];
   }
 }
 
-// tile products of a round: kind 0 (a <= b): G[Ia, Ib] <- Qa^T G[Ia, Ib] Qb,
-// and its transpose into G[Ib, Ia];
-// kind 1: V[rows r0..r0+63, Ib] <- V[.., Ib] Qb
+// tile products of a round on v_mfma_f64_16x16x4 (lane l holds A[l%16][l/16],
+// B[l/16][l%16]; result register r holds D[l/16 + 4r][l%16]).  Wave w
+// computes rows 16w..16w+15 of the 64 x 64 result; the operand all waves
+// share is staged in LDS, the per-wave operand is read from memory.
+//   kind 0 (a <= b): X = Qa^T T, R = X Qb for T = G[Ia, Ib]; R written to
+//                    G[Ia, Ib] and mirrored into G[Ib, Ia];
+//   kind 1: R^T = Qb^T T^T for T = V[rows 64a..64a+63, Ib]: V <- V Qb, the
+//           transposed product so that lanes store down V's columns.
+constexpr int kTLd = kSub + 1;
 __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const int4* items, int nitems,
-                                                    int r, const int* active, double* Gp, double* Vp,
-                                                    const double* Qp) {
-  __shared__ double Ts[kSub * kSubLd];
-  __shared__ double Qa[kSub * kSubLd];
-  __shared__ double Qb[kSub * kSubLd];
+                                                    int r, const int* active, const int* rotated,
+                                                    double* Gp, double* Vp, const double* Qp) {
+  __shared__ double Bs[kSub * kTLd];
+  __shared__ double Xs[kSub * kTLd];
   const int4 it = items[blockIdx.x];
   const int li = it.x;
   const BJSector S = sec[li];
   if (!active[li] || r >= S.nbk - 1) return;
   const int kind = it.w, a = it.y, b = it.z;
-  const int tid = threadIdx.x;
+  const bool rb = rotated[S.pb + b] != 0;
+  const bool ra = kind == 0 && rotated[S.pb + a] != 0;
+  if (!ra && !rb) return;   // both factors are the identity
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int l16 = lane & 15, l4 = lane >> 4;
   const double* qa = Qp + S.qp + (int64_t)a * kSub * kSub;
   const double* qb = Qp + S.qp + (int64_t)b * kSub * kSub;
-  double* M = kind == 0 ? Gp : Vp;
-  // load: rows (kind 0: pair a's indices; kind 1: a*64 + t), columns: pair b's indices
+  double af[16];
+  bj_v4d acc[4];
+  if (kind == 1) {
+    double* V = Vp + S.gp;
+#pragma unroll
+    for (int s = 0; s < 16; s++) af[s] = qb[(4 * s + l4) + kSub * (16 * w + l16)];   // Qb^T
+    for (int e = tid; e < kSub * kSub; e += 256) {   // Bs[k][ii] = V(64a + ii, gj(k))
+      const int ii = e % kSub, kk = e / kSub;
+      Bs[kk * kTLd + ii] = V[(int64_t)(a * kSub + ii) + (int64_t)S.N * bj_index(S.nbk, r, b, kk)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 4; f++) acc[f] = bj_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 16; s++)
+#pragma unroll
+      for (int f = 0; f < 4; f++)
+        acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], Bs[(4 * s + l4) * kTLd + 16 * f + l16], acc[f],
+                                                      0, 0, 0);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const int64_t col = (int64_t)S.N * bj_index(S.nbk, r, b, 16 * w + l4 + 4 * rr);
+#pragma unroll
+      for (int f = 0; f < 4; f++) V[col + a * kSub + 16 * f + l16] = acc[f][rr];
+    }
+    return;
+  }
+  double* G = Gp + S.gp;
+#pragma unroll
+  for (int s = 0; s < 16; s++) af[s] = qa[(4 * s + l4) + kSub * (16 * w + l16)];   // Qa^T
   for (int e = tid; e < kSub * kSub; e += 256) {
-    const int i = e % kSub, j = e / kSub;
-    const int gi = kind == 0 ? bj_index(S.nbk, r, a, i) : a * kSub + i;
-    const int gj = bj_index(S.nbk, r, b, j);
-    Ts[i * kSubLd + j] = M[S.gp + gi + (int64_t)S.N * gj];
-    Qb[i * kSubLd + j] = qb[i + kSub * j];
-    if (kind == 0) Qa[i * kSubLd + j] = qa[i + kSub * j];
+    const int kk = e % kSub, j = e / kSub;
+    Bs[kk * kTLd + j] = G[bj_index(S.nbk, r, a, kk) + (int64_t)S.N * bj_index(S.nbk, r, b, j)];
+    Xs[kk * kTLd + j] = qb[kk + kSub * j];
   }
   __syncthreads();
-  // U = T Qb: thread owns rows i0..i0+3, columns j0..j0+3
-  const int i0 = (tid / 16) * 4, j0 = (tid % 16) * 4;
-  double u[4][4];
-  for (int x = 0; x < 4; x++)
-    for (int y = 0; y < 4; y++) u[x][y] = 0.0;
-  for (int kk = 0; kk < kSub; kk++) {
-    double tv[4], qv[4];
-    for (int x = 0; x < 4; x++) tv[x] = Ts[(i0 + x) * kSubLd + kk];
-    for (int y = 0; y < 4; y++) qv[y] = Qb[kk * kSubLd + j0 + y];
-    for (int x = 0; x < 4; x++)
-      for (int y = 0; y < 4; y++) u[x][y] += tv[x] * qv[y];
-  }
-  if (kind == 0) {   // R = Qa^T U
-    __syncthreads();
-    for (int x = 0; x < 4; x++)
-      for (int y = 0; y < 4; y++) Ts[(i0 + x) * kSubLd + j0 + y] = u[x][y];
-    __syncthreads();
-    for (int x = 0; x < 4; x++)
-      for (int y = 0; y < 4; y++) u[x][y] = 0.0;
-    for (int kk = 0; kk < kSub; kk++) {
-      double qv[4], tv[4];
-      for (int x = 0; x < 4; x++) qv[x] = Qa[kk * kSubLd + i0 + x];
-      for (int y = 0; y < 4; y++) tv[y] = Ts[kk * kSubLd + j0 + y];
-      for (int x = 0; x < 4; x++)
-        for (int y = 0; y < 4; y++) u[x][y] += qv[x] * tv[y];
+#pragma unroll
+  for (int f = 0; f < 4; f++) acc[f] = bj_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 16; s++)
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+      acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], Bs[(4 * s + l4) * kTLd + 16 * f + l16], acc[f],
+                                                    0, 0, 0);
+  __syncthreads();   // every wave is done reading T
+#pragma unroll
+  for (int f = 0; f < 4; f++)
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) Bs[(16 * w + l4 + 4 * rr) * kTLd + 16 * f + l16] = acc[f][rr];
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 16; s++) af[s] = Bs[(16 * w + l16) * kTLd + 4 * s + l4];   // X rows of wave w
+#pragma unroll
+  for (int f = 0; f < 4; f++) acc[f] = bj_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 16; s++)
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+      acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], Xs[(4 * s + l4) * kTLd + 16 * f + l16], acc[f],
+                                                    0, 0, 0);
+#pragma unroll
+  for (int rr = 0; rr < 4; rr++) {
+    const int gi = bj_index(S.nbk, r, a, 16 * w + l4 + 4 * rr);
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+      const int gj = bj_index(S.nbk, r, b, 16 * f + l16);
+      G[gi + (int64_t)S.N * gj] = acc[f][rr];
+      if (a != b) G[gj + (int64_t)S.N * gi] = acc[f][rr];   // G stays symmetric
     }
   }
-  for (int x = 0; x < 4; x++)
-    for (int y = 0; y < 4; y++) {
-      const int gi = kind == 0 ? bj_index(S.nbk, r, a, i0 + x) : a * kSub + i0 + x;
-      const int gj = bj_index(S.nbk, r, b, j0 + y);
-      M[S.gp + gi + (int64_t)S.N * gj] = u[x][y];
-      if (kind == 0 && a != b) M[S.gp + gj + (int64_t)S.N * gi] = u[x][y];   // G stays symmetric
-    }
 }
 
 __global__ void k_bjac_finish(const BJSector* sec, int nsec, const int64_t* goff,
@@ -672,7 +735,7 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
     const int q = big[li], n = nq[q];
     const int N = (n + kSub - 1) / kSub * kSub;
     const int nbk = N / kBJ, npair = nbk / 2;
-    sec[li] = BJSector{n, N, nbk, q, gp, qp, 0.0, 0.0};
+    sec[li] = BJSector{n, N, nbk, q, gp, qp, 0.0, 0.0, (int)pitems.size(), 0};
     gp += (int64_t)N * N;
     qp += (int64_t)npair * kSub * kSub;
     maxr = std::max(maxr, nbk - 1);
@@ -701,10 +764,11 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   double* Qp = dalloc<double>(qp);
   int* act = dalloc<int>(ns);
   int* flag = dalloc<int>(ns);
+  int* rot = dalloc<int>(pitems.size());
   auto release = [&]() {
-    dfree(d_sec); dfree(d_pit); dfree(d_tit); dfree(Gp); dfree(Vp); dfree(Qp); dfree(act); dfree(flag);
+    dfree(d_sec); dfree(d_pit); dfree(d_tit); dfree(Gp); dfree(Vp); dfree(Qp); dfree(act); dfree(flag); dfree(rot);
   };
-  if (!d_sec || !d_pit || !d_tit || !Gp || !Vp || !Qp || !act || !flag) {
+  if (!d_sec || !d_pit || !d_tit || !Gp || !Vp || !Qp || !act || !flag || !rot) {
     release();
     set_error("svd: block Jacobi allocation failed");
     return 1;
@@ -714,14 +778,16 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   TTG_HIP(hipMemcpyAsync(d_pit, pitems.data(), sizeof(int2) * pitems.size(), hipMemcpyHostToDevice, s));
   TTG_HIP(hipMemcpyAsync(d_tit, titems.data(), sizeof(int4) * titems.size(), hipMemcpyHostToDevice, s));
   TTG_HIP(hipMemcpyAsync(act, hact.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
+  TTG_HIP(hipFuncSetAttribute((const void*)k_bjac_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kSolveLds));
   hipLaunchKernelGGL(k_bjac_init, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, G, Gp, Vp);
   for (int sweep = 0; sweep < max_sweeps; sweep++) {
     TTG_HIP(hipMemsetAsync(flag, 0, sizeof(int) * ns, s));
     for (int r = 0; r < maxr; r++) {
-      hipLaunchKernelGGL(k_bjac_solve, dim3((unsigned)pitems.size()), dim3(kSolveThreads), 0, s, d_sec, d_pit,
-                         (int)pitems.size(), r, act, flag, Gp, Qp);
+      hipLaunchKernelGGL(k_bjac_solve, dim3((unsigned)pitems.size()), dim3(kSolveThreads), kSolveLds, s, d_sec, d_pit,
+                         (int)pitems.size(), r, act, flag, rot, Gp, Qp);
       hipLaunchKernelGGL(k_bjac_apply, dim3((unsigned)titems.size()), dim3(256), 0, s, d_sec, d_tit,
-                         (int)titems.size(), r, act, Gp, Vp, Qp);
+                         (int)titems.size(), r, act, rot, Gp, Vp, Qp);
     }
     std::vector<int> hf(ns);
     TTG_HIP(hipMemcpyAsync(hf.data(), flag, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
