@@ -208,6 +208,7 @@ __global__ void k_svd_gather(MetaView A, int nleft, const double* __restrict__ a
 // ~sqrt(eps)), and the rounding noise in the null space of a rank-deficient
 // sector's Gram matrix never passes the relative test alone.
 constexpr int kJacThreads = 1024;
+constexpr double kJacFloor = 1e-14;   // convergence floor relative to the largest diagonal
 constexpr int kLdsDoubles = 20000;   // dynamic LDS (160 000 B of the CU's 160 KiB)
 
 __device__ inline int pk(int i, int j) { return i <= j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j; }
@@ -247,7 +248,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
          const double* __restrict__ Gin, double* __restrict__ gscratch,
          const int64_t* __restrict__ poff, double* __restrict__ Vout,
          double* __restrict__ evals, const int64_t* __restrict__ eoff, int max_sweeps,
-         int nmax_here) {
+         int nmax_here, double floor_rel) {
   extern __shared__ double lds_g[];
   __shared__ int any_rot;
   __shared__ double red[kJacThreads / 64];
@@ -294,7 +295,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
   __syncthreads();
   dmax = 0.0;
   for (int i = 0; i < (int)(blockDim.x >> 6); i++) dmax = fmax(dmax, red[i]);
-  const double floor_abs = 1e-14 * dmax;
+  const double floor_abs = floor_rel * dmax;
   const int m1 = np - 1;
   for (int sweep = 0; sweep < max_sweeps; sweep++) {
     if (threadIdx.x == 0) any_rot = 0;
@@ -383,7 +384,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
 // with nothing to rotate leaves Q_IJ = I and marks its pair unrotated; the
 // tile products of a round skip the tiles whose pairs were both unrotated.
 constexpr int kJacBlockMin = 64;
-constexpr double kBJRel = 1e-12, kBJFloor = 1e-14;
+constexpr double kBJRel = 1e-12;
 constexpr int kBJ = 32;                // block width; subproblems are 2 kBJ
 constexpr int kSub = 2 * kBJ;
 constexpr int kSubLd = kSub + 1;       // LDS row pitch (bank spread)
@@ -723,7 +724,7 @@ __global__ void k_bjac_finish(const BJSector* sec, int nsec, const int64_t* goff
 static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
                         const std::vector<int64_t>& goff_h, const int64_t* d_goff,
                         const int64_t* d_eoff, const double* G, double* Vev, double* ev,
-                        hipStream_t s, int max_sweeps) {
+                        hipStream_t s, int max_sweeps, double floor_rel) {
   const int ns = (int)big.size();
   std::vector<BJSector> sec(ns);
   std::vector<int2> pitems;
@@ -746,7 +747,7 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
       for (int b = 0; b < npair; b++) titems.push_back(int4{li, rc, b, 1});
   }
   // largest diagonal per sector (host copy of the Gram diagonals)
-  for (int li = 0; li < ns; li++) {
+  for (int li = 0; li < ns && floor_rel > 0; li++) {
     const int q = big[li], n = nq[q];
     std::vector<double> d(n);
     TTG_HIP(hipMemcpy2DAsync(d.data(), sizeof(double), G + goff_h[q], sizeof(double) * (n + 1),
@@ -754,7 +755,7 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
     TTG_HIP(hipStreamSynchronize(s));
     for (double v : d) dmax[li] = std::max(dmax[li], std::fabs(v));
     sec[li].dmax = dmax[li];
-    sec[li].floor = kBJFloor * dmax[li];
+    sec[li].floor = floor_rel * dmax[li];
   }
   BJSector* d_sec = dalloc<BJSector>(ns);
   int2* d_pit = dalloc<int2>(pitems.size());
@@ -802,6 +803,175 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   TTG_HIP(hipStreamSynchronize(s));
   release();
   return 0;
+}
+
+// 4c. LR preconditioning of the large sectors (cutoff > 0).  A sector's
+// Gram matrix of a DMRG state is numerically rank-deficient (a few hundred
+// of its rows carry weight), and the Jacobi iteration on G spends most of
+// its sweeps decoupling that null space from the rest.  A pivoted Cholesky
+// factorisation P^T G P = R^T R stops at the numerical rank r (largest
+// remaining diagonal below 1e-15 of the largest diagonal) and gives
+// G' = R R^T (r x r, the same nonzero eigenvalues, no null space).  Jacobi
+// on G' with the relative convergence test alone (no absolute floor: G' is
+// positive definite) converges in about a third of the sweeps, at size r,
+// and G's eigenvectors follow as V = R^T W Lambda^(-1/2), orthonormal to the
+// relative accuracy of W.  The n - r null-space columns are left zero with
+// eigenvalue 0: a zero singular value is never kept when cutoff > 0
+// (qtensor_op.cpp:1294-1300).
+// R is stored by original column index (row k of R at k n + j), so neither
+// G' nor V needs the permutation.
+//   k_pchol_init   d = diag(G), no column pivoted yet, dmax per sector
+//   k_pchol_panel  one workgroup per sector: nb pivot steps (largest
+//                  remaining diagonal, ties to the lower index), each row of R
+//                  from the trailing matrix minus the panel's earlier rows
+//                  (left-looking inside the panel, rows held in LDS)
+//   k_pchol_update trailing matrix G <- G - R_panel^T R_panel (rank nb)
+constexpr double kPcTol = 1e-15;
+constexpr int kPcThreads = 1024;
+struct PcSector {
+  int n, q;
+  int64_t goff;   // G (trailing matrix, overwritten) and R, n x n each
+  int64_t doff;   // diagonal / pivot flags, n each
+};
+
+__global__ void __launch_bounds__(kPcThreads) k_pchol_init(const PcSector* ps, const double* G, double* d,
+                                                           int* piv, int* rank, double* dmax) {
+  __shared__ double red[kPcThreads / 64];
+  const PcSector P = ps[blockIdx.x];
+  double mx = 0.0;
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
+    const double v = G[P.goff + j + (int64_t)P.n * j];
+    d[P.doff + j] = v;
+    piv[P.doff + j] = 0;
+    mx = fmax(mx, v);
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+    dmax[blockIdx.x] = m;
+    rank[blockIdx.x] = -1;
+  }
+}
+
+// dynamic LDS: nb panel rows (nb x n), d (n), pivot flags (n ints)
+__global__ void __launch_bounds__(kPcThreads) k_pchol_panel(const PcSector* ps, int k0, int nb,
+                                                            const double* A, double* R, double* dg,
+                                                            int* pivg, int* rank, const double* dmax) {
+  extern __shared__ __attribute__((aligned(16))) double pc_lds[];
+  __shared__ double rv[kPcThreads / 64];
+  __shared__ int ri[kPcThreads / 64];
+  __shared__ int s_p, s_stop;
+  __shared__ double s_rkk;
+  const PcSector P = ps[blockIdx.x];
+  const int n = P.n, tid = threadIdx.x;
+  if (k0 >= n || rank[blockIdx.x] >= 0) return;   // finished sector
+  double* Rp = pc_lds;                 // panel row s at s * n
+  double* d = pc_lds + (int64_t)nb * n;
+  int* piv = (int*)(d + n);
+  for (int j = tid; j < n; j += blockDim.x) {
+    d[j] = dg[P.doff + j];
+    piv[j] = pivg[P.doff + j];
+  }
+  __syncthreads();
+  const double tol = kPcTol * dmax[blockIdx.x];
+  const int kend = min(k0 + nb, n);
+  int k = k0;
+  for (; k < kend; k++) {
+    double bv = -1.0;
+    int bi = n;
+    for (int j = tid; j < n; j += blockDim.x)
+      if (!piv[j] && (d[j] > bv || (d[j] == bv && j < bi))) { bv = d[j]; bi = j; }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o);
+      const int oi = __shfl_xor(bi, o);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double v = rv[0];
+      int i = ri[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+        if (rv[w] > v || (rv[w] == v && ri[w] < i)) { v = rv[w]; i = ri[w]; }
+      s_stop = !(v > tol) || i >= n;
+      s_p = i;
+      s_rkk = s_stop ? 0.0 : sqrt(v);
+      if (!s_stop) piv[i] = 1;
+    }
+    __syncthreads();
+    if (s_stop) break;
+    const int p = s_p, ks = k - k0;
+    const double rkk = s_rkk;
+    const double irkk = 1.0 / rkk;
+    for (int j = tid; j < n; j += blockDim.x) {
+      double v;
+      if (j == p) {
+        v = rkk;
+      } else if (piv[j]) {
+        v = 0.0;
+      } else {
+        v = A[P.goff + j + (int64_t)n * p];   // trailing matrix, column p (symmetric)
+        for (int t = 0; t < ks; t++) v -= Rp[t * n + p] * Rp[t * n + j];
+        v *= irkk;
+        d[j] -= v * v;
+      }
+      Rp[ks * n + j] = v;
+      R[P.goff + (int64_t)k * n + j] = v;
+    }
+    __syncthreads();
+  }
+  for (int j = tid; j < n; j += blockDim.x) {
+    dg[P.doff + j] = d[j];
+    pivg[P.doff + j] = piv[j];
+  }
+  if (tid == 0 && k < kend) rank[blockIdx.x] = k;   // stopped inside this panel
+}
+
+// G[i, j] -= sum_{k in panel rows} R[k, i] R[k, j], 64 x 64 tiles, 256 threads
+__global__ void __launch_bounds__(256) k_pchol_update(const PcSector* ps, int k0, int nb, const int* rank,
+                                                      double* A, const double* R) {
+  const PcSector P = ps[blockIdx.y];
+  const int n = P.n;
+  const int nt = (n + 63) / 64;
+  if ((int)blockIdx.x >= nt * nt || k0 >= n) return;
+  const int rk = rank[blockIdx.y];
+  const int k1 = rk >= 0 ? min(rk, min(k0 + nb, n)) : min(k0 + nb, n);
+  if (k1 <= k0) return;
+  const int i0 = (blockIdx.x % nt) * 64, j0 = (blockIdx.x / nt) * 64;
+  const int i = i0 + (threadIdx.x & 63);
+  const int jb = j0 + (threadIdx.x >> 6) * 16;
+  if (i >= n) return;
+  double acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) acc[u] = 0.0;
+  for (int k = k0; k < k1; k++) {
+    const double* rk_ = R + P.goff + (int64_t)k * n;
+    const double ri_ = rk_[i];
+#pragma unroll
+    for (int u = 0; u < 16; u++)
+      if (jb + u < n) acc[u] += ri_ * rk_[jb + u];
+  }
+#pragma unroll
+  for (int u = 0; u < 16; u++)
+    if (jb + u < n) A[P.goff + i + (int64_t)n * (jb + u)] -= acc[u];
+}
+
+// V columns c < r: (R^T W)_c / sqrt(lambda_c), eigenvalue lambda_c; columns
+// c >= r: zero, eigenvalue 0
+__global__ void k_lr_finish(const PcSector* ps, const int* rk, const int64_t* eoff2, const double* ev2,
+                            const int64_t* eoff, double* Vev, double* ev) {
+  const PcSector P = ps[blockIdx.y];
+  const int n = P.n, r = rk[blockIdx.y];
+  const int c = blockIdx.x;
+  if (c >= n) return;
+  const double lam = c < r ? ev2[eoff2[blockIdx.y] + c] : 0.0;
+  const double sc = (c < r && lam > 0.0) ? 1.0 / sqrt(lam) : 0.0;
+  double* col = Vev + P.goff + (int64_t)n * c;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = c < r ? col[i] * sc : 0.0;
+  if (threadIdx.x == 0) ev[eoff[P.q] + c] = lam;
 }
 
 // 5a. per sector: eigen order (descending), top-k singular values in the
@@ -1041,6 +1211,114 @@ static T* up(const std::vector<T>& v) {
   return d;
 }
 
+// eigenpairs of the large sectors `big` through the LR step (see 4c)
+static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, const std::vector<int64_t>& goff,
+                    const std::vector<int64_t>& eoff, const int64_t* d_eoff, int64_t gtotal, double* G,
+                    double* Vev, double* ev, hipStream_t s) {
+  const int ns = (int)big.size();
+  std::vector<PcSector> ps(ns);
+  int nmax = 0;
+  int64_t dtot = 0;
+  for (int li = 0; li < ns; li++) {
+    const int q = big[li];
+    ps[li] = PcSector{nq[q], q, goff[q], dtot};
+    dtot += nq[q];
+    nmax = std::max(nmax, nq[q]);
+  }
+  // panel rows: nb rows of n doubles, d and the flags in the 160 KiB of LDS
+  const int nb = std::max(1, std::min(16, (int)(163840 / (8 * (int64_t)nmax)) - 2));
+  const size_t lds = sizeof(double) * ((size_t)nb * nmax + nmax) + sizeof(int) * nmax;
+  TTG_REQUIRE(lds <= 163840, "svd: sector too large for the pivoted Cholesky panel");
+  std::vector<void*> tmp;
+  auto release = [&]() { for (void* p : tmp) dfree(p); };
+  PcSector* d_ps = dalloc<PcSector>(ns);
+  double* R = dalloc<double>(gtotal);
+  double* dg = dalloc<double>(dtot);
+  int* pg = dalloc<int>(dtot);
+  int* rank = dalloc<int>(ns);
+  double* dmax = dalloc<double>(ns);
+  for (void* p : std::initializer_list<void*>{d_ps, R, dg, pg, rank, dmax}) tmp.push_back(p);
+  if (!d_ps || !R || !dg || !pg || !rank || !dmax) { release(); set_error("svd: allocation failed"); return 1; }
+  TTG_HIP(hipMemcpyAsync(d_ps, ps.data(), sizeof(PcSector) * ns, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_pchol_init, dim3(ns), dim3(kPcThreads), 0, s, d_ps, G, dg, pg, rank, dmax);
+  TTG_HIP(hipFuncSetAttribute((const void*)k_pchol_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  const int nt = (nmax + 63) / 64;
+  for (int k0 = 0; k0 < nmax; k0 += nb) {
+    hipLaunchKernelGGL(k_pchol_panel, dim3(ns), dim3(kPcThreads), lds, s, d_ps, k0, nb, G, R, dg, pg, rank,
+                       dmax);
+    hipLaunchKernelGGL(k_pchol_update, dim3(nt * nt, ns), dim3(256), 0, s, d_ps, k0, nb, rank, G, R);
+  }
+  std::vector<int> rk(ns);
+  TTG_HIP(hipMemcpyAsync(rk.data(), rank, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  std::vector<int> r2(ns);
+  std::vector<int64_t> goff2(ns + 1, 0), poff2(ns + 1, 0), eoff2(ns + 1, 0);
+  int64_t lds_need = 1;
+  bool any_small = false;
+  std::vector<int> big2;
+  for (int li = 0; li < ns; li++) {
+    const int r = rk[li] < 0 ? ps[li].n : rk[li];
+    r2[li] = r;
+    goff2[li + 1] = goff2[li] + (int64_t)r * r;
+    const int64_t np = r + (r & 1);
+    poff2[li + 1] = poff2[li] + np * (np + 1) / 2;
+    eoff2[li + 1] = eoff2[li] + r;
+    if (r > kJacBlockMin) big2.push_back(li);
+    else if (r > 0) { any_small = true; lds_need = std::max(lds_need, jac_layout(r).total); }
+  }
+  double* G2 = dalloc<double>(std::max<int64_t>(1, goff2[ns]));
+  double* W2 = dalloc<double>(std::max<int64_t>(1, goff2[ns]));
+  double* gs2 = dalloc<double>(std::max<int64_t>(1, poff2[ns]));
+  double* ev2 = dalloc<double>(std::max<int64_t>(1, eoff2[ns]));
+  int* d_r2 = up(r2);
+  int64_t* d_goff2 = up(goff2);
+  int64_t* d_poff2 = up(poff2);
+  int64_t* d_eoff2 = up(eoff2);
+  for (void* p : std::initializer_list<void*>{G2, W2, gs2, ev2, d_r2, d_goff2, d_poff2, d_eoff2}) tmp.push_back(p);
+  // G' = R R^T: A(k, j) = R[k n + j], B(j, k') = R[k' n + j]
+  {
+    HostGemm hg;
+    for (int li = 0; li < ns; li++) {
+      const int n = ps[li].n, r = r2[li];
+      if (r == 0) continue;
+      hg.tasks.push_back(gtask(ps[li].goff, ps[li].goff, r, n, n, 1, 1, r, n, n));
+      const int t = (int)hg.tasks.size() - 1;
+      hg.cblk.push_back(CBlock{goff2[li], r, r, t, t + 1});
+    }
+    if (run_host_gemm(hg, R, R, G2)) { release(); return 1; }
+  }
+  // eigenpairs of G' (relative convergence test only)
+  if (any_small) {
+    const size_t lb = sizeof(double) * (size_t)std::min<int64_t>(lds_need, kLdsDoubles);
+    TTG_HIP(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double) * kLdsDoubles)));
+    hipLaunchKernelGGL(k_jacobi, dim3(ns), dim3(256), lb, s, d_r2, d_goff2, G2, gs2, d_poff2, W2, ev2,
+                       d_eoff2, 40, kJacBlockMin, 0.0);
+    TTG_HIP(hipGetLastError());
+  }
+  if (!big2.empty() && block_jacobi(big2, r2, goff2, d_goff2, d_eoff2, G2, W2, ev2, s, 40, 0.0)) {
+    release();
+    return 1;
+  }
+  // V = R^T W Lambda^(-1/2): A(j, k) = R[k n + j], B(k, c) = W[k + r c]
+  {
+    HostGemm hv;
+    for (int li = 0; li < ns; li++) {
+      const int n = ps[li].n, r = r2[li];
+      if (r == 0) continue;
+      hv.tasks.push_back(gtask(ps[li].goff, goff2[li], n, 1, r, n, 1, r, r, r));
+      const int t = (int)hv.tasks.size() - 1;
+      hv.cblk.push_back(CBlock{ps[li].goff, n, r, t, t + 1});
+    }
+    if (run_host_gemm(hv, R, W2, Vev)) { release(); return 1; }
+  }
+  hipLaunchKernelGGL(k_lr_finish, dim3(nmax, ns), dim3(256), 0, s, d_ps, d_r2, d_eoff2, ev2, d_eoff, Vev, ev);
+  TTG_HIP(hipStreamSynchronize(s));
+  release();
+  return 0;
+}
+
 }  // namespace ttg
 
 using namespace ttg;
@@ -1099,7 +1377,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   std::vector<int> nq(nQ), kq(nQ);
   for (int q = 0; q < nQ; q++) {
     const int64_t n = fromLeft ? rows[q] : cols[q];
-    TTG_REQUIRE(n <= 1024, "svd: sector larger than 1024 (Jacobi pair table)");
+    TTG_REQUIRE(n <= 8192, "svd: sector larger than 8192");
     nq[q] = (int)n;
     int kd = (int)std::min(rows[q], cols[q]);
     kq[q] = K > 0 ? (int)std::min<int64_t>(K, kd) : kd;   // rank-K SVD per sector
@@ -1171,10 +1449,14 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     // small sectors: fewer waves per workgroup (cheaper barriers per round)
     const int jac_threads = nsmall <= 96 ? 256 : kJacThreads;
     hipLaunchKernelGGL(k_jacobi, dim3(nQ), dim3(jac_threads), lds_bytes, s, d_nq, d_goff, G, gs,
-                       d_poff, Vev, ev, d_eoff, 40, kJacBlockMin);
+                       d_poff, Vev, ev, d_eoff, 40, kJacBlockMin, kJacFloor);
     TTG_HIP(hipGetLastError());
   }
-  if (!big.empty() && block_jacobi(big, nq, goff, d_goff, d_eoff, G, Vev, ev, s, 40)) {
+  // large sectors: the LR step when the truncation drops zero singular
+  // values (cutoff > 0), else block Jacobi on G
+  if (!big.empty() &&
+      (cutoff > 0 ? lr_eigen(big, nq, goff, eoff, d_eoff, goff[nQ], G, Vev, ev, s)
+                  : block_jacobi(big, nq, goff, d_goff, d_eoff, G, Vev, ev, s, 40, kJacFloor))) {
     release();
     return 1;
   }
