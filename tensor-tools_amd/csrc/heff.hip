@@ -35,8 +35,8 @@ struct ttg_heff_s {
   double flops = 0;
   ~ttg_heff_s() {
     for (auto* q : p) delete q;
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     ttg::dfree(T1); ttg::dfree(T2);
     ttg::dfree(L32); ttg::dfree(W32); ttg::dfree(R32); ttg::dfree(x32);
     ttg::dfree(xbuf); ttg::dfree(ybuf);
@@ -101,10 +101,10 @@ static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
   return 0;
 }
 
-// Measured tile-shape choice for an MFMA stage of an operator: each MFMA
-// tile shape's work list is built and its launch timed (one untimed, two
-// timed, HIP events on the library stream) on the stage's own operands; the
-// fastest shape is kept.  The choice is cached by a coarse signature of the
+// Measured tile-shape choice for an MFMA stage of an operator: each
+// candidate tile shape's work list is built and its launch timed (one
+// untimed, one timed, HIP events on the library stream) on the stage's own
+// operands; the fastest shape is kept.  The choice is cached by a coarse signature of the
 // stage (precision, stage, rounded log2 flops and tasks, operand layout), so
 // the operators of neighbouring DMRG bonds reuse it without timing.  Stages
 // the planner gave the vector-ALU kernel, stages below kTuneMinFlops (their
@@ -134,19 +134,24 @@ static int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void*
   TTG_HIP(hipEventCreate(&e1));
   int best = p->cfg;
   float best_ms = 1e30f;
-  for (int c = 0; c < TILE_NCFG; c++) {
-    if (rebuild_tiles(*p, c)) { hipEventDestroy(e0); hipEventDestroy(e1); return 1; }
+  // candidates: the analytic choice and the shapes that win on the DMRG
+  // stages (32x32, 64x32, 96x32, 32x96: narrow tiles, many work items)
+  std::vector<int> cand{p->cfg};
+  for (int c : {TILE_32x32, TILE_64x32, TILE_96x32, TILE_32x96})
+    if (c != p->cfg) cand.push_back(c);
+  for (int c : cand) {
+    if (rebuild_tiles(*p, c)) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return 1; }
     TTG_TRY(gemm_launch(*p, A, B, C, out_f64, s));
     TTG_HIP(hipEventRecord(e0, s));
-    for (int rep = 0; rep < 2; rep++) TTG_TRY(gemm_launch(*p, A, B, C, out_f64, s));
+    TTG_TRY(gemm_launch(*p, A, B, C, out_f64, s));
     TTG_HIP(hipEventRecord(e1, s));
     TTG_HIP(hipEventSynchronize(e1));
     float ms = 0;
     TTG_HIP(hipEventElapsedTime(&ms, e0, e1));
     if (ms < best_ms) { best_ms = ms; best = c; }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   g_tuned[key] = best;
   return rebuild_tiles(*p, best);
 }
@@ -239,7 +244,7 @@ int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_lik
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(s, &g);
   if (rr || e != hipSuccess) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     delete h;
     set_error("heff: graph capture failed");
     return 1;
@@ -295,8 +300,8 @@ int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms) {
   TTG_HIP(hipEventSynchronize(e1));
   float t = 0;
   TTG_HIP(hipEventElapsedTime(&t, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   mark_written(y);
   *ms = t / n;
   return 0;
@@ -331,7 +336,7 @@ int ttg_heff_stage_time(ttg_heff h, ttg_tensor x, int n, double* ms3) {
       acc[k] += t;
     }
   }
-  for (auto& e : ev) hipEventDestroy(e);
+  for (auto& e : ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 3; k++) ms3[k] = acc[k] / n;
   return 0;
 }
@@ -346,7 +351,7 @@ int ttg_heff_tiles(ttg_heff h, int* tile_cfg3, int* ntiles3) {
 }
 
 int ttg_heff_free(ttg_heff h) {
-  if (h) hipStreamSynchronize(stream());
+  if (h) TTG_HIP(hipStreamSynchronize(stream()));
   delete h;
   return 0;
 }

@@ -661,7 +661,7 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
     dfree(uL); dfree(uR); dfree(uM);
     dfree(q.qL0); dfree(q.qL1); dfree(q.qR0); dfree(q.qR1); dfree(q.qM0); dfree(q.qM1);
     dfree(q.qbase); dfree(d_hdr);
-    hipHostFree(h_hdr);
+    (void)hipHostFree(h_hdr);
   };
 
   hipLaunchKernelGGL(k_plan_keys, dim3((nmax + 255) / 256), dim3(256), 0, s, A->view(), B->view(),

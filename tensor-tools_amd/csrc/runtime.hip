@@ -44,8 +44,8 @@ void* dmalloc(size_t bytes) {
   } else {
     if (hipMalloc(&p, sz) != hipSuccess) {
       // release cached buffers and retry once
-      hipDeviceSynchronize();
-      for (auto& kv : g_pool.free_list) hipFree(kv.second);
+      (void)hipDeviceSynchronize();
+      for (auto& kv : g_pool.free_list) (void)hipFree(kv.second);
       g_pool.free_list.clear();
       g_pool.cached = 0;
       if (hipMalloc(&p, sz) != hipSuccess) {
@@ -102,7 +102,7 @@ int ttg_init(int device) {
 int ttg_finalize(void) {
   if (!g_rt.inited) return 0;
   TTG_HIP(hipStreamSynchronize(g_rt.stream));
-  for (auto& kv : g_pool.free_list) hipFree(kv.second);
+  for (auto& kv : g_pool.free_list) (void)hipFree(kv.second);
   g_pool.free_list.clear();
   g_pool.cached = 0;
   return 0;

@@ -387,7 +387,6 @@ constexpr int kJacBlockMin = 64;
 constexpr double kBJRel = 1e-12;
 constexpr int kBJ = 32;                // block width; subproblems are 2 kBJ
 constexpr int kSub = 2 * kBJ;
-constexpr int kSubLd = kSub + 1;       // LDS row pitch (bank spread)
 
 typedef double bj_v4d __attribute__((ext_vector_type(4)));
 
@@ -588,17 +587,30 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
   // 63 rounds return every index to its first slot; the result is in buffer 1.
   // Eigenpairs in descending order (the larger values to the pair's lower
   // block): a fixed order keeps values from swapping between blocks from
-  // round to round, which otherwise holds the outer iteration at a linear rate
+  // round to round, which otherwise holds the outer iteration at a linear rate.
+  // Only the sector's own indices are ordered, over their own positions; a
+  // padding index (global index >= n, never rotated) keeps its column, so no
+  // eigenvector moves into the padding that k_bjac_finish drops.
   const double* Sf = bj_lds + kSlotMat;
   const double* Qf = bj_lds + 3 * kSlotMat;
   if (tid < kSub) {
-    const double d = Sf[tid * kSP + tid];
-    int rk = 0;
-    for (int u = 0; u < kSub; u++) {
-      const double du = Sf[u * kSP + u];
-      rk += (du > d) || (du == d && u < tid);
+    const int x = tid, p = slot_of(x);
+    int dst = x;
+    if (bj_index(S.nbk, r, k, x) < S.n) {
+      const double d = Sf[p * kSP + p];
+      int rk = 0;
+      for (int u = 0; u < kSub; u++) {
+        if (bj_index(S.nbk, r, k, u) >= S.n) continue;
+        const int pu = slot_of(u);
+        const double du = Sf[pu * kSP + pu];
+        rk += (du > d) || (du == d && u < x);
+      }
+      for (int u = 0; u < kSub; u++) {   // the rk-th own index in ascending order
+        if (bj_index(S.nbk, r, k, u) >= S.n) continue;
+        if (rk-- == 0) { dst = u; break; }
+      }
     }
-    perm[tid] = rk;
+    perm[p] = dst;   // column of Q for the eigenvector in slot p
   }
   __syncthreads();
   for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
@@ -1204,10 +1216,16 @@ static GemmTask gtask(int64_t a_off, int64_t b_off, int em0, int64_t sam0, int e
   return g;
 }
 
+// device copy of a host vector (nullptr, with the error set, on failure)
 template <class T>
 static T* up(const std::vector<T>& v) {
   T* d = dalloc<T>(std::max<size_t>(v.size(), 1));
-  if (!v.empty()) hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream());
+  if (d && !v.empty() &&
+      hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream()) != hipSuccess) {
+    set_error("svd: host-to-device copy failed");
+    dfree(d);
+    return nullptr;
+  }
   return d;
 }
 
@@ -1276,6 +1294,11 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
   int64_t* d_poff2 = up(poff2);
   int64_t* d_eoff2 = up(eoff2);
   for (void* p : std::initializer_list<void*>{G2, W2, gs2, ev2, d_r2, d_goff2, d_poff2, d_eoff2}) tmp.push_back(p);
+  if (!G2 || !W2 || !gs2 || !ev2 || !d_r2 || !d_goff2 || !d_poff2 || !d_eoff2) {
+    release();
+    set_error("svd: allocation failed");
+    return 1;
+  }
   // G' = R R^T: A(k, j) = R[k n + j], B(j, k') = R[k' n + j]
   {
     HostGemm hg;
@@ -1412,9 +1435,17 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   int64_t* d_selo = up(selo);
   int* d_nq = up(nq);
   int* d_kq = up(kq);
+  bool alloc_ok = true;
   for (void* p : std::initializer_list<void*>{mat, G, Vev, gs, ev, Esel, svals, ssec, order, ssorted, newdim, tout,
-                                              d_moff, d_goff, d_poff, d_eoff, d_soff, d_selo, d_nq, d_kq})
+                                              d_moff, d_goff, d_poff, d_eoff, d_soff, d_selo, d_nq, d_kq}) {
     scratch.push_back(p);
+    alloc_ok = alloc_ok && p;
+  }
+  if (!alloc_ok) {
+    release();
+    set_error("svd: device allocation failed");
+    return 1;
+  }
   // ---- 2. gather
   fill_f64(mat, moff[nQ], 0.0);
   hipLaunchKernelGGL(k_svd_gather, dim3(std::min(nb, 4096)), dim3(256), 0, s, m.view(), nleft,

@@ -141,10 +141,12 @@ bool same_structure(ttg_tensor a, ttg_tensor b) {
   if (x.nblocks != y.nblocks || x.total != y.total || !x.same_indices(y)) return false;
   // compare block lists on the host (small)
   std::vector<int> qa((size_t)x.nblocks * x.rank), qb(qa.size());
-  if (!qa.empty()) {
-    hipMemcpyAsync(qa.data(), x.d_qi, qa.size() * sizeof(int), hipMemcpyDeviceToHost, stream());
-    hipMemcpyAsync(qb.data(), y.d_qi, qb.size() * sizeof(int), hipMemcpyDeviceToHost, stream());
-    hipStreamSynchronize(stream());
+  if (!qa.empty() &&
+      (hipMemcpyAsync(qa.data(), x.d_qi, qa.size() * sizeof(int), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
+       hipMemcpyAsync(qb.data(), y.d_qi, qb.size() * sizeof(int), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
+       hipStreamSynchronize(stream()) != hipSuccess)) {
+    set_error("block list comparison: device copy failed");
+    return false;
   }
   return qa == qb;
 }
