@@ -24,6 +24,9 @@
 // Operand lane layout (k-slice of 4):  lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; result register r of lane l holds D[row][l%16] with
 // row = l/16 + 4r for f64 and 4*(l/16) + r for the f32-accumulating ops.
+// The kernels pass the B fragment as the first operand and the A fragment as
+// the second, so D is the transposed tile (B^T A^T): lane l holds output
+// row l%16, and a column of C is stored by 16 consecutive lanes.
 #include <algorithm>
 #include <unordered_map>
 
@@ -248,23 +251,26 @@ k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cb
           for (int i = 0; i < FM; i++)
 #pragma unroll
             for (int j = 0; j < FN; j++)
-              acc[i][j] = MfmaOp<T>::mac(af[i][kk >> 1][kk & 1], bf[j][kk >> 1][kk & 1], acc[i][j]);
+              acc[i][j] = MfmaOp<T>::mac(bf[j][kk >> 1][kk & 1], af[i][kk >> 1][kk & 1], acc[i][j]);
         if (left > 0) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
       }
     }
-    // epilogue: C block is column-major M x N; a split piece writes its
-    // whole TM x TN partial (column-major) to its slot instead
+    // epilogue: the fragments hold the transposed tile (B^T A^T: the B
+    // fragment goes in as the MFMA's first operand), so lane l has rows
+    // m = l % 16 of output columns n = row(l, r) and the 16 lanes of a column
+    // store 128 contiguous bytes of the column-major M x N C block; a split
+    // piece writes its whole TM x TN partial (column-major) to its slot
     T* pt = tl.slot >= 0 ? part + (int64_t)tl.slot * (TM * TN) : nullptr;
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int nl = wn * (TN / WN) + j * 16 + r16, n = tl.n0 + nl;
+        const int ml = wm * (TM / WM) + i * 16 + r16, m = tl.m0 + ml;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int ml = wm * (TM / WM) + i * 16 + MfmaOp<T>::row(lane, r), m = tl.m0 + ml;
+          const int nl = wn * (TN / WN) + j * 16 + MfmaOp<T>::row(lane, r), n = tl.n0 + nl;
           if (pt) pt[ml + TM * nl] = acc[i][j][r];
           else if (m < cb.M && n < cb.N) C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)acc[i][j][r];
         }
@@ -434,24 +440,25 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
         for (int i = 0; i < FM; i++)
 #pragma unroll
           for (int j = 0; j < FN; j++) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
           }
         if (left > 0) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
       }
     }
+    // transposed fragments (see k_gemm): lane l holds rows m = l % 16
     float* pt = tl.slot >= 0 ? part + (int64_t)tl.slot * (TM * TN) : nullptr;
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int nl = wn * (TN / WN) + j * 16 + (lane & 15), n = tl.n0 + nl;
+        const int ml = wm * (TM / WM) + i * 16 + (lane & 15), m = tl.m0 + ml;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int ml = wm * (TM / WM) + i * 16 + 4 * (lane >> 4) + r, m = tl.m0 + ml;
+          const int nl = wn * (TN / WN) + j * 16 + 4 * (lane >> 4) + r, n = tl.n0 + nl;
           if (pt) pt[ml + TM * nl] = acc[i][j][r];
           else if (m < cb.M && n < cb.N) C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)acc[i][j][r];
         }
