@@ -46,8 +46,8 @@ def stage_traffic(stage, cfg):
         d = json.load(open(path))
     except (OSError, ValueError):
         return None
-    rec = d.get("stages", {}).get(str(stage + 1))
-    if not rec or rec.get("work_shape") != cfg:
+    rec = d.get("stages", {}).get(str(stage + 1), {}).get(str(cfg))
+    if not rec:
         return None
     return int(round((rec["fetch_kib"] * d["fetch_scale"] + rec["write_kib"] * d["write_scale"]) * 1024))
 
