@@ -82,6 +82,8 @@ def sweep_leg(P, T, args):
                          total_s=round(r["seconds"], 4), E=r["E"], E_per_site=r["E"] / N,
                          n_matvec=r["n_matvec"], t_davidson=round(r["t_davidson"], 4),
                          t_svd=round(r["t_svd"], 4), t_env=round(r["t_env"], 4),
+                         trunc_err_max=r["trunc_err_max"],
+                         trunc_err_last_halfsweep=r["trunc_err_last_halfsweep"],
                          max_bond=int(np.max(psi.bond_dims())))
         log(f"[bench] sweep {name}: {sweep_s:.3f} s (halves {res[name]['halfsweep_s']}, "
             f"davidson {r['t_davidson']:.2f} s, svd {r['t_svd']:.2f} s, env {r['t_env']:.2f} s, "
@@ -91,6 +93,8 @@ def sweep_leg(P, T, args):
     if "fp64" in res:
         for name, v in res.items():
             v["dE_per_site_vs_fp64"] = abs(v["E_per_site"] - res["fp64"]["E_per_site"])
+            v["dtrunc_last_halfsweep_vs_fp64"] = abs(v["trunc_err_last_halfsweep"]
+                                                    - res["fp64"]["trunc_err_last_halfsweep"])
     out = dict(m=args.sweep_m, sites=N, cutoff=1e-9, max_restart=2, forms=res)
     if args.cpu_seconds > 0 and args.sweep_m == m_w:
         out["cpu_baseline"] = cpu_bond_sample(args, N)

@@ -303,6 +303,13 @@ double tth_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int 
       stats[0] = log.t_dav; stats[1] = log.t_svd; stats[2] = log.t_env;
       stats[3] = (double)log.n_matvec;
       stats[4] = std::chrono::duration<double>(t1 - t0).count();
+      // largest truncation error of any bond, and the sum over the last half-sweep's bonds
+      double tmax = 0, tlast = 0;
+      for (double t : log.trunc_err) tmax = std::max(tmax, t);
+      const size_t nb = log.trunc_err.size(), per = psi ? ((qMPS<double>*)psi)->length - 1 : 0;
+      for (size_t i = (nb > per ? nb - per : 0); i < nb; i++) tlast += log.trunc_err[i];
+      stats[5] = tmax;
+      stats[6] = tlast;
     }
     return E;
   )

@@ -309,8 +309,9 @@ def dmrg(psi: PMPS, H: PMPO, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12
     hs = np.zeros(2 * nsweeps)
     hsec = np.zeros(2 * nsweeps)
     e0 = _d(0)
-    stats = np.zeros(5)
+    stats = np.zeros(7)
     E = _ok(_lib().tth_dmrg(psi.h, H.h, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr, tol, hs,
                             hsec, C.byref(e0), stats), "dmrg")
     return dict(E=E, halfsweep_E=hs, halfsweep_s=hsec, E0=e0.value, t_davidson=stats[0],
-                t_svd=stats[1], t_env=stats[2], n_matvec=int(stats[3]), seconds=stats[4])
+                t_svd=stats[1], t_env=stats[2], n_matvec=int(stats[3]), seconds=stats[4],
+                trunc_err_max=stats[5], trunc_err_last_halfsweep=stats[6])
