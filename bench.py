@@ -176,7 +176,7 @@ def main():
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus "
               f"{args.gpus}); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
