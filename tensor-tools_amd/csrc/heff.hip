@@ -135,9 +135,10 @@ static int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void*
   int best = p->cfg;
   float best_ms = 1e30f;
   // candidates: the analytic choice and the shapes that win on the DMRG
-  // stages (32x32, 64x32, 96x32, 32x96: narrow tiles, many work items)
+  // stages (32x32, 64x32, 32x64, 96x32, 32x96: narrow tiles, many work
+  // items; 96x96 for the split-BF16 L x stage)
   std::vector<int> cand{p->cfg};
-  for (int c : {TILE_32x32, TILE_64x32, TILE_96x32, TILE_32x96})
+  for (int c : {TILE_32x32, TILE_64x32, TILE_32x64, TILE_96x32, TILE_32x96, TILE_96x96})
     if (c != p->cfg) cand.push_back(c);
   for (int c : cand) {
     if (rebuild_tiles(*p, c)) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return 1; }
