@@ -530,8 +530,7 @@ static int choose_cfg(const PlanHdr& h, int prec, int ncu, bool am, bool bk) {
 // consecutive k ranges, each written to its own partial slot and summed in
 // slot order by the reduction pass (deterministic).  `piece` is half of one
 // workgroup's share of the whole walk, so that the longest item is short
-// against the launch.  Items are ordered longest first; equal lengths keep
-// block order so neighbouring tiles of one block share an XCD's L2.
+// against the launch.  Items keep output-block order.
 int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
   const int TM = cfg_tm(p.cfg), TN = cfg_tn(p.cfg);
   const int64_t G = 4 * (int64_t)rt().num_cu;
@@ -563,8 +562,11 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
       }
   }
   TTG_REQUIRE(tiles.size() < (size_t)INT32_MAX && nslots < INT32_MAX, "plan: work list too long");
-  std::stable_sort(tiles.begin(), tiles.end(),
-                   [](const Tile& a, const Tile& b) { return a.kt1 - a.kt0 > b.kt1 - b.kt0; });
+  // Items stay in output-block creation order (q -> right -> left): the
+  // tiles of one block share A rows, and consecutive blocks share B blocks,
+  // so consecutive items (one XCD's run, see xcd_first_item) reuse its L2.
+  // A longest-walk-first order balanced the tail no better and ran the
+  // m=2048 and m=8192 operators 1-4% slower.
   hipStream_t s = stream();
   p.ntiles = (int)tiles.size();
   p.ngroups = (int)groups.size();
