@@ -988,6 +988,21 @@ __global__ void k_lr_finish(const PcSector* ps, const int* rk, const int64_t* eo
 
 // 5a. per sector: eigen order (descending), top-k singular values in the
 // reference's sector / descending order, selected eigenvectors packed
+// ordering key of a value: NaN compares as -inf, so that rank counts over
+// any input form a permutation
+__device__ inline double rank_key(double v) { return v == v ? v : -INFINITY; }
+
+// every sector's Gram diagonal finite (a Gram matrix's entries are bounded by
+// its diagonal, and a NaN input reaches the diagonal); *bad = 1 otherwise
+__global__ void __launch_bounds__(256) k_svd_finite(int nQ, const int* nq, const int64_t* goff,
+                                                    const double* G, int* bad) {
+  const int q = blockIdx.x;
+  if (q >= nQ) return;
+  const int n = nq[q];
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    if (!isfinite(G[goff[q] + i + (int64_t)n * i])) *bad = 1;
+}
+
 __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq,
                                                     const int64_t* eoff, const double* evals,
                                                     const int64_t* goff, const double* V,
@@ -997,11 +1012,16 @@ __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq
   const int q = blockIdx.x;
   const int n = nq[q], k = kq[q];
   const double* ev = evals + eoff[q];
-  // rank of each eigenvalue (descending, ties by index): rank < k selected
+  // rank of each eigenvalue (descending, ties by index): rank < k selected.
+  // The ranks are a permutation for any input (a NaN ranks as -inf), so the
+  // slots below are each written once.
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = ev[i];
+    const double v = rank_key(ev[i]);
     int r = 0;
-    for (int j = 0; j < n; j++) r += (ev[j] > v) || (ev[j] == v && j < i);
+    for (int j = 0; j < n; j++) {
+      const double u = rank_key(ev[j]);
+      r += (u > v) || (u == v && j < i);
+    }
     if (r < k) {
       svals[soff[q] + r] = sqrt(fmax(v, 0.0));
       ssec[soff[q] + r] = q;
@@ -1021,17 +1041,20 @@ __global__ void __launch_bounds__(256) k_svd_rank(int T, const double* __restric
                                                   int* __restrict__ order, double* __restrict__ sorted) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= T) return;
-  const double v = svals[i];
+  const double v = rank_key(svals[i]);
   int r = 0;
-  for (int j = 0; j < T; j++) r += (svals[j] > v) || (svals[j] == v && j < i);
+  for (int j = 0; j < T; j++) {
+    const double u = rank_key(svals[j]);
+    r += (u > v) || (u == v && j < i);
+  }
   order[r] = i;
-  sorted[r] = v;
+  sorted[r] = svals[i];
 }
 
 // 5c. the reference's keep loop (qtensor_op.cpp:1294-1300), sequential in its
 // summation order: nm over (sector, descending) order, cumsum over the sorted
 // values
-__global__ void k_svd_truncate(int T, const double* svals, const int* ssec, const int* order,
+__global__ void k_svd_truncate(int T, int nQ, const double* svals, const int* ssec, const int* order,
                                const double* sorted, double cutoff, int64_t K, int* newdim,
                                double* out) {
   if (threadIdx.x != 0) return;
@@ -1043,7 +1066,10 @@ __global__ void k_svd_truncate(int T, const double* svals, const int* ssec, cons
     const double s = sorted[i];
     cumsum += s * s / nm;
     if (s == 0. && cutoff != 0.) break;
-    newdim[ssec[order[i]]] += 1;
+    const int oi = order[i];
+    const int q = (oi >= 0 && oi < T) ? ssec[oi] : -1;
+    if (q < 0 || q >= nQ) break;   // unreachable: order and ssec are filled by the kernels above
+    newdim[q] += 1;
     acc += 1;
     if ((1 - cumsum < cutoff && cutoff != 0.) || (acc >= K && K != 0)) break;
   }
@@ -1464,6 +1490,23 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     }
     if (run_host_gemm(hg, mat, mat, G)) { release(); return 1; }
   }
+  // a sector whose Gram matrix overflowed (entries of A above ~1e154) or that
+  // holds a NaN has no meaningful eigenpairs: fail before the eigensolver
+  {
+    int* bad = dalloc<int>(1);
+    if (!bad) { release(); set_error("svd: device allocation failed"); return 1; }
+    scratch.push_back(bad);
+    int hbad = 0;
+    TTG_HIP(hipMemcpyAsync(bad, &hbad, sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_svd_finite, dim3(nQ), dim3(256), 0, s, nQ, d_nq, d_goff, G, bad);
+    TTG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipStreamSynchronize(s));
+    if (hbad) {
+      release();
+      set_error("svd: a sector's Gram matrix is not finite (tensor entries above ~1e154, or NaN)");
+      return 1;
+    }
+  }
   // ---- 4. eigensolver
   std::vector<int> big;
   int64_t lds_need = 1;
@@ -1499,7 +1542,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     const int T = (int)soff[nQ];
     if (T > 0)
       hipLaunchKernelGGL(k_svd_rank, dim3((T + 255) / 256), dim3(256), 0, s, T, svals, order, ssorted);
-    hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(64), 0, s, T, svals, ssec, order, ssorted,
+    hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(64), 0, s, T, nQ, svals, ssec, order, ssorted,
                        cutoff, K, newdim, tout);
   }
   std::vector<int> nd(nQ);

@@ -435,6 +435,10 @@ double calcEntropy(qtensor<double>& S, double cutoff) {
   return e;
 }
 
+// progress lines on stderr every g_progress_every sites or bonds (0: silent)
+static int g_progress_every = 0;
+void set_dmrg_progress(int every) { g_progress_every = every; }
+
 // ---------------------------------------------------------------- canonical form
 static void rename_like(qtensor_index& ix, const qtensor_index& mid) {
   ix.rename(mid.name());
@@ -457,6 +461,11 @@ void qTensorTrain<double, N>::rc() {   // qtt.cpp:821-850
     rename_like(A[i].idx_set[0], mid);
     A[i - 1] = A[i - 1] * U;
     rename_like(A[i - 1].idx_set.back(), mid);
+    if (g_progress_every > 0 && (i % g_progress_every == 0 || i == 1)) {
+      ttg_sync();
+      fprintf(stderr, "[dmrg] right-canonical form: site %u/%u, bond dim %u\n", i, length - 1, bond_dims[i]);
+      fflush(stderr);
+    }
   }
   center = 0;
 }
@@ -671,9 +680,6 @@ double psiphi(qMPS<double>& psi, qMPS<double>& phi) {
   }
   return res;
 }
-
-static int g_progress_every = 0;
-void set_dmrg_progress(int every) { g_progress_every = every; }
 
 double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart, double tol,

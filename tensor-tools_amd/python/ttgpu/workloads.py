@@ -100,15 +100,26 @@ def mps_link_sectors(N: int, b: int, m: int, sigma: float):
 
 def random_mps(N: int, m: int, sigma: float, seed: int, mps_id: int = 1):
     """Site tensors psi_i = (ID<id>Link<i> In, Site<i> In, ID<id>Link<i+1> Out)
-    (qtt.cpp:246-253) over all legal blocks, N(0,1) from PCG64(seed + i)."""
+    (qtt.cpp:246-253) over all legal blocks, N(0,1) from PCG64(seed + i)
+    times c_i = 1 / sqrt(2 max_q D_q) over the sectors of link i.
+
+    The scale keeps the canonicalisation finite.  rc() (qtt.cpp:821-850)
+    carries psi_i * U_{i+1} leftwards unnormalised, and with N(0,1) entries
+    the carried squared norm of sector q grows by about
+    D_q(link i) * sum_s n_{q+s} per site: 1e330 at N = 128, m = 8192, past
+    the range of the FP64 Gram matrices.  With c_i it stays within
+    [1e-3, 1] at every size.  The normalised state is the same, since a
+    product of site scales is one global factor."""
     out = []
     for i in range(N):
-        li = make_index(INWARD, f"ID{mps_id}Link{i}", LINK, 0,
-                        [((q, 0), d) for q, d in mps_link_sectors(N, i, m, sigma)])
+        lsec = mps_link_sectors(N, i, m, sigma)
+        li = make_index(INWARD, f"ID{mps_id}Link{i}", LINK, 0, [((q, 0), d) for q, d in lsec])
         si = site_index(INWARD, f"Site{i}", 0)
         lo = make_index(OUTWARD, f"ID{mps_id}Link{i + 1}", LINK, 0,
                         [((q, 0), d) for q, d in mps_link_sectors(N, i + 1, m, sigma)])
-        out.append(fill([li, si, lo], seed + i))
+        t = fill([li, si, lo], seed + i)
+        t.data *= 1.0 / np.sqrt(2.0 * max(d for _, d in lsec))
+        out.append(t)
     return out
 
 
