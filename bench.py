@@ -53,8 +53,9 @@ def stage_traffic(stage, cfg):
 
 
 def sweep_leg(P, T, args):
-    """One full two-site DMRG sweep (both half-sweeps, N-1 bonds each) on the
-    workload's lattice, max bond dimension --sweep-m, SVD cutoff 1e-9,
+    """--sweeps full two-site DMRG sweeps (both half-sweeps, N-1 bonds each)
+    on the workload's lattice, max bond dimension --sweep-m, SVD cutoff
+    --sweep-cutoff (1e-9),
     Davidson max_restart 2, tol 1e-12, from the same seeded random MPS
     (Gaussian Sz sectors of width sigma, m per link where the Hilbert space
     allows) in every precision form.  Reduced-precision forms end each
@@ -75,10 +76,12 @@ def sweep_leg(P, T, args):
         for i, t in enumerate(tensors):
             psi.set_tensor(i, P.PTensor.from_flat(t))
         psi.set_center(-1)
-        r = P.dmrg(psi, H, 1, [args.sweep_m], [1e-9], [2], tol=1e-12)
-        sweep_s = float(np.sum(r["halfsweep_s"]))
+        r = P.dmrg(psi, H, args.sweeps, [args.sweep_m], [args.sweep_cutoff], [2], tol=1e-12)
+        hs = np.asarray(r["halfsweep_s"], dtype=float)
+        sweep_s = float(np.sum(hs)) / args.sweeps
         res[name] = dict(sweep_s=round(sweep_s, 4),
-                         halfsweep_s=[round(float(v), 4) for v in r["halfsweep_s"]],
+                         sweep_s_each=[round(float(v), 4) for v in hs.reshape(-1, 2).sum(axis=1)],
+                         halfsweep_s=[round(float(v), 4) for v in hs],
                          total_s=round(r["seconds"], 4), E=r["E"], E_per_site=r["E"] / N,
                          n_matvec=r["n_matvec"], t_davidson=round(r["t_davidson"], 4),
                          t_svd=round(r["t_svd"], 4), t_env=round(r["t_env"], 4),
@@ -95,8 +98,9 @@ def sweep_leg(P, T, args):
             v["dE_per_site_vs_fp64"] = abs(v["E_per_site"] - res["fp64"]["E_per_site"])
             v["dtrunc_last_halfsweep_vs_fp64"] = abs(v["trunc_err_last_halfsweep"]
                                                     - res["fp64"]["trunc_err_last_halfsweep"])
-    out = dict(m=args.sweep_m, sites=N, cutoff=1e-9, max_restart=2, forms=res)
-    if args.cpu_seconds > 0 and args.sweep_m == m_w:
+    out = dict(m=args.sweep_m, sites=N, sweeps=args.sweeps, cutoff=args.sweep_cutoff, max_restart=2,
+               forms=res)
+    if args.cpu_seconds > 0 and args.sweep_m == m_w and args.sweeps == 1 and args.sweep_cutoff == 1e-9:
         out["cpu_baseline"] = cpu_bond_sample(args, N)
     return out
 
@@ -171,6 +175,8 @@ def main():
     ap.add_argument("--sweep-m", type=int, default=2048,
                     help="bond dimension of the DMRG sweep leg (0 skips the sweep)")
     ap.add_argument("--sweep-forms", default="fp64,bf16x3,fp32")
+    ap.add_argument("--sweeps", type=int, default=1, help="sweeps of the sweep leg (sweep_s is per sweep)")
+    ap.add_argument("--sweep-cutoff", type=float, default=1e-9, help="SVD cutoff of the sweep leg")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a dispatch of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")

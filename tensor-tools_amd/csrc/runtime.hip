@@ -43,12 +43,16 @@ void* dmalloc(size_t bytes) {
     g_pool.free_list.erase(it);
   } else {
     if (hipMalloc(&p, sz) != hipSuccess) {
-      // release cached buffers and retry once
+      // release cached buffers and retry once.  The failure is handled here,
+      // so it is cleared from the runtime's last-error state: the next
+      // launch check (hipGetLastError) would otherwise report it.
+      (void)hipGetLastError();
       (void)hipDeviceSynchronize();
       for (auto& kv : g_pool.free_list) (void)hipFree(kv.second);
       g_pool.free_list.clear();
       g_pool.cached = 0;
       if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
         set_error("device allocation of " + std::to_string(sz) + " bytes failed");
         return nullptr;
       }
