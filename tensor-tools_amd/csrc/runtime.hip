@@ -1,4 +1,5 @@
 // runtime.hip -- device binding, stream, error string and caching pool.
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -15,12 +16,29 @@ Runtime& rt() { return g_rt; }
 // ---------------------------------------------------------------- pool
 // Size classes: round up to 256 B, then to a multiple of 1/8 of the next
 // power of two, so a freed buffer is reused by any request within ~12%.
+// The pool holds at most `budget` bytes (live plus cached): the device's free
+// memory at ttg_init less a reserve for the HIP runtime's own allocations
+// (queues, kernel scratch), which fail with the device full of cached buffers.
+// A request that would pass the budget first returns the cache to HIP, and
+// is refused (an error, not a device-wide failure) if live buffers alone
+// would pass it.
 namespace {
 struct Pool {
   std::multimap<size_t, void*> free_list;
   std::map<void*, size_t> live;
   size_t in_use = 0, peak = 0, cached = 0;
+  size_t budget = SIZE_MAX;
 } g_pool;
+
+// hand every cached buffer back to HIP (after the device is idle: a buffer
+// freed by the host may still be read by queued work)
+void release_cache() {
+  if (g_pool.free_list.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : g_pool.free_list) (void)hipFree(kv.second);
+  g_pool.free_list.clear();
+  g_pool.cached = 0;
+}
 
 size_t size_class(size_t b) {
   if (b < 256) return 256;
@@ -42,15 +60,19 @@ void* dmalloc(size_t bytes) {
     g_pool.cached -= sz;
     g_pool.free_list.erase(it);
   } else {
+    if (g_pool.in_use + g_pool.cached + sz > g_pool.budget) release_cache();
+    if (g_pool.in_use + sz > g_pool.budget) {
+      set_error("device allocation of " + std::to_string(sz) + " bytes refused: " +
+                std::to_string(g_pool.in_use >> 20) + " MiB live, budget " + std::to_string(g_pool.budget >> 20) +
+                " MiB (device memory less a reserve for the HIP runtime)");
+      return nullptr;
+    }
     if (hipMalloc(&p, sz) != hipSuccess) {
       // release cached buffers and retry once.  The failure is handled here,
       // so it is cleared from the runtime's last-error state: the next
       // launch check (hipGetLastError) would otherwise report it.
       (void)hipGetLastError();
-      (void)hipDeviceSynchronize();
-      for (auto& kv : g_pool.free_list) (void)hipFree(kv.second);
-      g_pool.free_list.clear();
-      g_pool.cached = 0;
+      release_cache();
       if (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
         set_error("device allocation of " + std::to_string(sz) + " bytes failed");
@@ -98,6 +120,10 @@ int ttg_init(int device) {
   TTG_HIP(hipGetDeviceProperties(&prop, device));
   g_rt.num_cu = prop.multiProcessorCount;
   TTG_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+  size_t mfree = 0, mtotal = 0;
+  TTG_HIP(hipMemGetInfo(&mfree, &mtotal));
+  const size_t reserve = std::max<size_t>(mtotal / 8, (size_t)8 << 30);
+  g_pool.budget = mfree > reserve ? mfree - reserve : mfree / 2;
   g_rt.device = device;
   g_rt.inited = true;
   return 0;
@@ -106,9 +132,7 @@ int ttg_init(int device) {
 int ttg_finalize(void) {
   if (!g_rt.inited) return 0;
   TTG_HIP(hipStreamSynchronize(g_rt.stream));
-  for (auto& kv : g_pool.free_list) (void)hipFree(kv.second);
-  g_pool.free_list.clear();
-  g_pool.cached = 0;
+  release_cache();
   return 0;
 }
 

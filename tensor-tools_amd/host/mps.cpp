@@ -718,10 +718,24 @@ double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vecto
                    search_space_size, max_restart[lmr], tol, &sl);
         double te = now_s();
         updateEnv(psi, H, TR, TL, site, direction);
+        // The environment this bond read on the far side is stale: the next
+        // half-sweep recomputes it before any bond reads it.  Releasing it
+        // keeps one environment set on the device instead of two (results
+        // are unchanged; the open-boundary ends TL[0] and TR[L-1] are kept).
+        if (direction == MoveFromLeft) {
+          if (site + 1 < (unsigned)L - 1) TR[site + 1] = qtensor<double>();
+          if (site == 0) TR[0] = qtensor<double>();
+        } else {
+          if (site - 1 >= 1) TL[site - 1] = qtensor<double>();
+          if (site == (unsigned)L - 1) TL[L - 1] = qtensor<double>();
+        }
         ttg_sync();
         if (g_progress_every > 0 && (i % g_progress_every == 0 || i == L - 2)) {
-          fprintf(stderr, "[dmrg] sweep %d half %d bond %d/%d E=%.12f kept=%u %.2fs\n", sweep,
-                  l % 2, i + 1, L - 1, sl.energy, sl.kept, now_s() - t0);
+          size_t in_use = 0, peak = 0, cached = 0;
+          ttg_mem_info(&in_use, &peak, &cached);
+          fprintf(stderr, "[dmrg] sweep %d half %d bond %d/%d E=%.12f kept=%u %.2fs (device pool %.1f GB live, "
+                  "%.1f GB cached)\n", sweep, l % 2, i + 1, L - 1, sl.energy, sl.kept, now_s() - t0, in_use / 1e9,
+                  cached / 1e9);
           fflush(stderr);
         }
         if (log) {
