@@ -528,12 +528,16 @@ static int choose_cfg(const PlanHdr& h, int prec, int ncu, bool am, bool bk) {
 // Work list of a plan.  Every output block is cut into TM x TN tiles; a tile
 // whose k walk is longer than `piece` k-tiles is split into S <= kMaxSplit
 // consecutive k ranges, each written to its own partial slot and summed in
-// slot order by the reduction pass (deterministic).  `piece` is half of one
-// workgroup's share of the whole walk, so that the longest item is short
-// against the launch.  Items keep output-block order.
+// slot order by the reduction pass (deterministic).  `piece` is the whole
+// walk over 6 x the CU count, so the longest item is short against the
+// launch while the partial tiles (written, then read by the reduction) stay
+// few: at 8 x the CU count the m=2048 operator's FP64 stage 3 spent ~30 us of
+// 116 us on them, at 6 x it runs in 88 us (whole matvec 0.253 ms against
+// 0.257-0.261 ms at 4 x and 5 x and 0.281 ms at 8 x; the m=8192 operator is
+// unchanged within 2%).  Items keep output-block order.
 int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
   const int TM = cfg_tm(p.cfg), TN = cfg_tn(p.cfg);
-  const int64_t G = 4 * (int64_t)rt().num_cu;
+  const int64_t G = 3 * (int64_t)rt().num_cu;
   int64_t work = 0;
   for (const CBlock& b : cb)
     work += (int64_t)((b.M + TM - 1) / TM) * ((b.N + TN - 1) / TN) * std::max(b.nkt, 1);
