@@ -1003,12 +1003,14 @@ __global__ void __launch_bounds__(256) k_svd_finite(int nQ, const int* nq, const
     if (!isfinite(G[goff[q] + i + (int64_t)n * i])) *bad = 1;
 }
 
+constexpr int kSelMax = 8192;   // largest sector (ttg_svd rejects larger ones)
 __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq,
                                                     const int64_t* eoff, const double* evals,
                                                     const int64_t* goff, const double* V,
                                                     const int64_t* soff, double* svals,
                                                     int* ssec, const int64_t* selo,
                                                     double* Esel) {
+  __shared__ int sel[kSelMax];   // sel[r]: the eigenpair of rank r < k
   const int q = blockIdx.x;
   const int n = nq[q], k = kq[q];
   const double* ev = evals + eoff[q];
@@ -1025,10 +1027,18 @@ __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq
     if (r < k) {
       svals[soff[q] + r] = sqrt(fmax(v, 0.0));
       ssec[soff[q] + r] = q;
-      const double* src = V + goff[q] + (int64_t)n * i;
-      double* dst = Esel + selo[q] + (int64_t)n * r;
-      for (int e = 0; e < n; e++) dst[e] = src[e];
+      sel[r] = i;
     }
+  }
+  __syncthreads();
+  // the selected eigenvectors, packed in rank order: consecutive threads
+  // copy consecutive rows of a column
+  const double* src = V + goff[q];
+  double* dst = Esel + selo[q];
+  const int64_t tot = (int64_t)n * k;
+  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int r = (int)(e / n), row = (int)(e - (int64_t)r * n);
+    dst[e] = src[row + (int64_t)n * sel[r]];
   }
 }
 
@@ -1054,27 +1064,63 @@ __global__ void __launch_bounds__(256) k_svd_rank(int T, const double* __restric
 // 5c. the reference's keep loop (qtensor_op.cpp:1294-1300), sequential in its
 // summation order: nm over (sector, descending) order, cumsum over the sorted
 // values
-__global__ void k_svd_truncate(int T, int nQ, const double* svals, const int* ssec, const int* order,
-                               const double* sorted, double cutoff, int64_t K, int* newdim,
-                               double* out) {
-  if (threadIdx.x != 0) return;
+constexpr int kTrChunk = 2048;   // values staged in LDS per pass of the serial loops
+__global__ void __launch_bounds__(256) k_svd_truncate(int T, int nQ, const double* svals, const int* ssec,
+                                                      const int* order, const double* sorted, double cutoff,
+                                                      int64_t K, int* newdim, double* out) {
+  __shared__ double buf[kTrChunk];
+  __shared__ double s_cum;
+  __shared__ long long s_acc;
+  __shared__ int s_done;
+  const int tid = threadIdx.x;
+  // nm: sequential sum in the (sector, descending) order.  The workgroup
+  // stages each chunk in LDS so that thread 0's serial loop reads no global
+  // memory (a load per iteration made the loop latency-bound).
   double nm = 0.0;
-  for (int i = 0; i < T; i++) nm += svals[i] * svals[i];
-  double cumsum = 0.0;
-  int64_t acc = 0;
-  for (int i = 0; i < T; i++) {
-    const double s = sorted[i];
-    cumsum += s * s / nm;
-    if (s == 0. && cutoff != 0.) break;
+  for (int c0 = 0; c0 < T; c0 += kTrChunk) {
+    const int cn = min(kTrChunk, T - c0);
+    for (int e = tid; e < cn; e += blockDim.x) buf[e] = svals[c0 + e];
+    __syncthreads();
+    if (tid == 0)
+      for (int e = 0; e < cn; e++) nm += buf[e] * buf[e];
+    __syncthreads();
+  }
+  if (tid == 0) { s_cum = 0.0; s_acc = 0; s_done = 0; }
+  __syncthreads();
+  // the keep loop over the sorted values, in order, chunk by chunk
+  for (int c0 = 0; c0 < T; c0 += kTrChunk) {
+    if (s_done) break;   // read after a barrier: the same value in every thread
+    const int cn = min(kTrChunk, T - c0);
+    for (int e = tid; e < cn; e += blockDim.x) buf[e] = sorted[c0 + e];
+    __syncthreads();
+    if (tid == 0) {
+      double cumsum = s_cum;
+      long long acc = s_acc;
+      int done = 0;
+      for (int e = 0; e < cn; e++) {
+        const double v = buf[e];
+        cumsum += v * v / nm;
+        if (v == 0. && cutoff != 0.) { done = 1; break; }
+        acc += 1;
+        if ((1 - cumsum < cutoff && cutoff != 0.) || (acc >= K && K != 0)) { done = 1; break; }
+      }
+      s_cum = cumsum;
+      s_acc = acc;
+      s_done = done;
+    }
+    __syncthreads();
+  }
+  // kept values per sector: the first acc entries of the sorted order
+  const long long acc = s_acc;
+  for (long long i = tid; i < acc; i += blockDim.x) {
     const int oi = order[i];
     const int q = (oi >= 0 && oi < T) ? ssec[oi] : -1;
-    if (q < 0 || q >= nQ) break;   // unreachable: order and ssec are filled by the kernels above
-    newdim[q] += 1;
-    acc += 1;
-    if ((1 - cumsum < cutoff && cutoff != 0.) || (acc >= K && K != 0)) break;
+    if (q >= 0 && q < nQ) atomicAdd(&newdim[q], 1);   // unreachable otherwise: order and ssec are complete
   }
-  out[0] = fmax(1 - cumsum, 0.0);
-  out[1] = (double)acc;
+  if (tid == 0) {
+    out[0] = fmax(1 - s_cum, 0.0);
+    out[1] = (double)acc;
+  }
 }
 
 // 6a. block lists of U (left..., mid) and V (mid, right...) and S (mid)
@@ -1542,7 +1588,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     const int T = (int)soff[nQ];
     if (T > 0)
       hipLaunchKernelGGL(k_svd_rank, dim3((T + 255) / 256), dim3(256), 0, s, T, svals, order, ssorted);
-    hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(64), 0, s, T, nQ, svals, ssec, order, ssorted,
+    hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(256), 0, s, T, nQ, svals, ssec, order, ssorted,
                        cutoff, K, newdim, tout);
   }
   std::vector<int> nd(nQ);
