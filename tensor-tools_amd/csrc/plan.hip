@@ -606,6 +606,17 @@ int rebuild_tiles(ttg_plan_s& p, int cfg) {
   return build_tiles(p, hcb);
 }
 
+// pinned host copy of a plan header, allocated once for the process: plans
+// are built one at a time, and every read of it follows a stream sync
+static PlanHdr* pinned_hdr() {
+  static PlanHdr* h = nullptr;
+  if (!h && hipHostMalloc((void**)&h, sizeof(PlanHdr)) != hipSuccess) {
+    (void)hipGetLastError();
+    h = nullptr;
+  }
+  return h;
+}
+
 int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
                int force_cfg, ttg_plan* out) {
   TTG_TRY(ensure_init());
@@ -656,8 +667,8 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   q.qR1 = dalloc<int>(nmax); q.qM0 = dalloc<int>(nmax); q.qM1 = dalloc<int>(nmax);
   q.qbase = dalloc<int64_t>(nmax + 1);
   PlanHdr* d_hdr = dalloc<PlanHdr>(1);
-  PlanHdr* h_hdr = nullptr;
-  TTG_HIP(hipHostMalloc((void**)&h_hdr, sizeof(PlanHdr)));
+  PlanHdr* h_hdr = pinned_hdr();
+  TTG_REQUIRE(h_hdr, "plan: pinned host header allocation failed");
   TTG_HIP(hipMemsetAsync(d_hdr, 0, sizeof(PlanHdr), s));
 
   auto release = [&]() {
@@ -667,7 +678,6 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
     dfree(uL); dfree(uR); dfree(uM);
     dfree(q.qL0); dfree(q.qL1); dfree(q.qR0); dfree(q.qR1); dfree(q.qM0); dfree(q.qM1);
     dfree(q.qbase); dfree(d_hdr);
-    (void)hipHostFree(h_hdr);
   };
 
   hipLaunchKernelGGL(k_plan_keys, dim3((nmax + 255) / 256), dim3(256), 0, s, A->view(), B->view(),

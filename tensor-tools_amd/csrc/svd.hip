@@ -412,6 +412,26 @@ __device__ inline int bj_index(int nbk, int r, int k, int t) {
   return (t >= kBJ ? b : a) * kBJ + (t & (kBJ - 1));
 }
 
+// sec[li].dmax = max_i |G_ii| of the sector, sec[li].floor = floor_rel * dmax
+__global__ void __launch_bounds__(256) k_bjac_floor(BJSector* sec, int nsec, const int64_t* goff, const double* G,
+                                                    double floor_rel) {
+  __shared__ double red[4];
+  const int li = blockIdx.x;
+  if (li >= nsec) return;
+  const int q = sec[li].q, n = sec[li].n;
+  const double* g = G + goff[q];
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, fabs(g[i + (int64_t)n * i]));
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    sec[li].dmax = m;
+    sec[li].floor = floor_rel * m;
+  }
+}
+
 __global__ void k_bjac_init(const BJSector* sec, int nsec, const int64_t* goff, const double* G,
                             double* Gp, double* Vp) {
   const int li = blockIdx.y;
@@ -733,8 +753,7 @@ __global__ void k_bjac_finish(const BJSector* sec, int nsec, const int64_t* goff
 }
 
 // host driver of the block Jacobi over the sectors `big` (sizes nq)
-static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
-                        const std::vector<int64_t>& goff_h, const int64_t* d_goff,
+static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq, const int64_t* d_goff,
                         const int64_t* d_eoff, const double* G, double* Vev, double* ev,
                         hipStream_t s, int max_sweeps, double floor_rel) {
   const int ns = (int)big.size();
@@ -743,7 +762,6 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   std::vector<int4> titems;
   int64_t gp = 0, qp = 0;
   int maxr = 0;
-  std::vector<double> dmax(ns, 0.0);
   for (int li = 0; li < ns; li++) {
     const int q = big[li], n = nq[q];
     const int N = (n + kSub - 1) / kSub * kSub;
@@ -757,17 +775,6 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
       for (int b = a; b < npair; b++) titems.push_back(int4{li, a, b, 0});   // a <= b, mirrored
     for (int rc = 0; rc < N / kSub; rc++)
       for (int b = 0; b < npair; b++) titems.push_back(int4{li, rc, b, 1});
-  }
-  // largest diagonal per sector (host copy of the Gram diagonals)
-  for (int li = 0; li < ns && floor_rel > 0; li++) {
-    const int q = big[li], n = nq[q];
-    std::vector<double> d(n);
-    TTG_HIP(hipMemcpy2DAsync(d.data(), sizeof(double), G + goff_h[q], sizeof(double) * (n + 1),
-                             sizeof(double), n, hipMemcpyDeviceToHost, s));
-    TTG_HIP(hipStreamSynchronize(s));
-    for (double v : d) dmax[li] = std::max(dmax[li], std::fabs(v));
-    sec[li].dmax = dmax[li];
-    sec[li].floor = floor_rel * dmax[li];
   }
   BJSector* d_sec = dalloc<BJSector>(ns);
   int2* d_pit = dalloc<int2>(pitems.size());
@@ -793,6 +800,9 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   TTG_HIP(hipMemcpyAsync(act, hact.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
   TTG_HIP(hipFuncSetAttribute((const void*)k_bjac_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kSolveLds));
+  // largest diagonal per sector and the convergence floor, on the device
+  if (floor_rel > 0)
+    hipLaunchKernelGGL(k_bjac_floor, dim3(ns), dim3(256), 0, s, d_sec, ns, d_goff, G, floor_rel);
   hipLaunchKernelGGL(k_bjac_init, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, G, Gp, Vp);
   for (int sweep = 0; sweep < max_sweeps; sweep++) {
     TTG_HIP(hipMemsetAsync(flag, 0, sizeof(int) * ns, s));
@@ -1392,7 +1402,7 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
                        d_eoff2, 40, kJacBlockMin, 0.0);
     TTG_HIP(hipGetLastError());
   }
-  if (!big2.empty() && block_jacobi(big2, r2, goff2, d_goff2, d_eoff2, G2, W2, ev2, s, 40, 0.0)) {
+  if (!big2.empty() && block_jacobi(big2, r2, d_goff2, d_eoff2, G2, W2, ev2, s, 40, 0.0)) {
     release();
     return 1;
   }
@@ -1576,7 +1586,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   // values (cutoff > 0), else block Jacobi on G
   if (!big.empty() &&
       (cutoff > 0 ? lr_eigen(big, nq, goff, eoff, d_eoff, goff[nQ], G, Vev, ev, s)
-                  : block_jacobi(big, nq, goff, d_goff, d_eoff, G, Vev, ev, s, 40, kJacFloor))) {
+                  : block_jacobi(big, nq, d_goff, d_eoff, G, Vev, ev, s, 40, kJacFloor))) {
     release();
     return 1;
   }
