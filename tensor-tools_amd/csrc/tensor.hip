@@ -126,7 +126,7 @@ int tensor_from_meta(const MetaPtr& m, bool zero, ttg_tensor* out) {
   t->d_data = dalloc<double>(m->total);
   if (!t->d_data) {
     delete t;
-    set_error("tensor: device allocation failed");
+    set_error(std::string("tensor: ") + ttg_last_error());   // the pool's reason
     return 1;
   }
   if (zero && m->total)
