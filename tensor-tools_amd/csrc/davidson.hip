@@ -132,13 +132,15 @@ int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out) {
   hipStream_t s = stream();
   const int nx = a.nblocks, ny = b.nblocks;
   const int grid = std::max(1, std::min(nx, 512));
-  uint64_t* keys = dalloc<uint64_t>(std::max(ny, 1));
-  uint64_t* skeys = dalloc<uint64_t>(std::max(ny, 1));
-  int* ids = dalloc<int>(std::max(ny, 1));
-  int* sids = dalloc<int>(std::max(ny, 1));
-  int* map = dalloc<int>(std::max(nx, 1));
-  int* err = dalloc<int>(1);
-  double* part = dalloc<double>(grid + 1);
+  Scratch sc;
+  uint64_t* keys = sc.get<uint64_t>(std::max(ny, 1));
+  uint64_t* skeys = sc.get<uint64_t>(std::max(ny, 1));
+  int* ids = sc.get<int>(std::max(ny, 1));
+  int* sids = sc.get<int>(std::max(ny, 1));
+  int* map = sc.get<int>(std::max(nx, 1));
+  int* err = sc.get<int>(1);
+  double* part = sc.get<double>(grid + 1);
+  TTG_TRY(sc.ok("dot_matched"));
   double r = 0.0;
   if (nx && ny) {
     hipLaunchKernelGGL(k_tuple_keys, dim3((ny + 255) / 256), dim3(256), 0, s, b.view(), keys, ids);
@@ -151,7 +153,6 @@ int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out) {
     TTG_HIP(hipMemcpyAsync(&r, part + grid, sizeof(double), hipMemcpyDeviceToHost, s));
   }
   TTG_HIP(hipStreamSynchronize(s));
-  dfree(keys); dfree(skeys); dfree(ids); dfree(sids); dfree(map); dfree(err); dfree(part);
   *out = r;
   return 0;
 }
@@ -159,13 +160,14 @@ int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out) {
 int ttg_small_eigh(int n, const double* M, double* evals, double* evecs) {
   TTG_TRY(ensure_init());
   TTG_REQUIRE(n >= 1 && n <= 16, "small_eigh: 1 <= n <= 16");
-  double* d = dalloc<double>(n * n * 2 + n);
+  Scratch sc;
+  double* d = sc.get<double>(n * n * 2 + n);
+  TTG_TRY(sc.ok("small_eigh"));
   TTG_HIP(hipMemcpyAsync(d, M, sizeof(double) * n * n, hipMemcpyHostToDevice, stream()));
   hipLaunchKernelGGL(k_small_eigh, dim3(1), dim3(64), 0, stream(), n, d, d + 2 * n * n, d + n * n);
   TTG_HIP(hipMemcpyAsync(evecs, d + n * n, sizeof(double) * n * n, hipMemcpyDeviceToHost, stream()));
   TTG_HIP(hipMemcpyAsync(evals, d + 2 * n * n, sizeof(double) * n, hipMemcpyDeviceToHost, stream()));
   TTG_HIP(hipStreamSynchronize(stream()));
-  dfree(d);
   return 0;
 }
 
@@ -174,12 +176,14 @@ int ttg_embed(ttg_tensor src, ttg_tensor dst) {
   TTG_REQUIRE(src->m->same_indices(*dst->m), "embed: index sectors differ");
   hipStream_t s = stream();
   const int ns = src->m->nblocks, nd = dst->m->nblocks;
-  uint64_t* keys = dalloc<uint64_t>(std::max(nd, 1));
-  uint64_t* skeys = dalloc<uint64_t>(std::max(nd, 1));
-  int* ids = dalloc<int>(std::max(nd, 1));
-  int* sids = dalloc<int>(std::max(nd, 1));
-  int* map = dalloc<int>(std::max(ns, 1));
-  int* err = dalloc<int>(1);
+  Scratch sc;
+  uint64_t* keys = sc.get<uint64_t>(std::max(nd, 1));
+  uint64_t* skeys = sc.get<uint64_t>(std::max(nd, 1));
+  int* ids = sc.get<int>(std::max(nd, 1));
+  int* sids = sc.get<int>(std::max(nd, 1));
+  int* map = sc.get<int>(std::max(ns, 1));
+  int* err = sc.get<int>(1);
+  TTG_TRY(sc.ok("embed"));
   int herr = 0;
   TTG_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
   fill_f64(dst->d_data, dst->m->total, 0.0);
@@ -195,7 +199,6 @@ int ttg_embed(ttg_tensor src, ttg_tensor dst) {
   }
   TTG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   TTG_HIP(hipStreamSynchronize(s));
-  dfree(keys); dfree(skeys); dfree(ids); dfree(sids); dfree(map); dfree(err);
   mark_written(dst);
   TTG_REQUIRE(herr == 0, "embed: source block missing from destination block list");
   return 0;

@@ -79,11 +79,13 @@ __global__ void k_remap(MetaView c, const uint64_t* xkeys, const int* xids, int 
 static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
   hipStream_t s = stream();
   const int nx = xm->nblocks, nc = p3->C->nblocks;
-  uint64_t* keys = dalloc<uint64_t>(std::max(nx, 1));
-  uint64_t* skeys = dalloc<uint64_t>(std::max(nx, 1));
-  int* ids = dalloc<int>(std::max(nx, 1));
-  int* sids = dalloc<int>(std::max(nx, 1));
-  int* err = dalloc<int>(1);
+  Scratch sc;
+  uint64_t* keys = sc.get<uint64_t>(std::max(nx, 1));
+  uint64_t* skeys = sc.get<uint64_t>(std::max(nx, 1));
+  int* ids = sc.get<int>(std::max(nx, 1));
+  int* sids = sc.get<int>(std::max(nx, 1));
+  int* err = sc.get<int>(1);
+  TTG_TRY(sc.ok("heff"));
   int herr = 0;
   TTG_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
   if (nx) {
@@ -95,7 +97,6 @@ static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
                        nx, xm->d_off, (const int*)nullptr, p3->d_cblk, err);
   TTG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   TTG_HIP(hipStreamSynchronize(s));
-  dfree(keys); dfree(skeys); dfree(ids); dfree(sids); dfree(err);
   TTG_REQUIRE(herr == 0, "heff: the product has blocks outside x's block list "
                          "(give x the full legal block list, ttg_tensor_init_blocks)");
   return 0;
@@ -224,7 +225,11 @@ int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_lik
   fill_f64(h->ybuf, h->xm->total, 0.0);   // blocks of x never produced stay zero
   if (red) {
     h->x32 = dalloc<float>(h->xm->total);
-    if (to_f32(L, &h->L32) || to_f32(W12, &h->W32) || to_f32(R, &h->R32)) { delete h; return 1; }
+    if (!h->x32 || to_f32(L, &h->L32) || to_f32(W12, &h->W32) || to_f32(R, &h->R32)) {
+      delete h;
+      set_error("heff: allocation failed");
+      return 1;
+    }
   }
   hipStream_t s = stream();
   {   // measured tile shapes of the MFMA stages (before the graph is captured)
@@ -263,7 +268,7 @@ int ttg_heff_new_output(ttg_heff h, ttg_tensor* y) {
 
 int ttg_heff_apply(ttg_heff h, ttg_tensor x, ttg_tensor y) {
   TTG_REQUIRE(h && x && y, "heff_apply: null argument");
-  TTG_REQUIRE(x->m->total == h->xm->total && y->m->total == h->xm->total,
+  TTG_REQUIRE(same_meta(x->m, h->xm) && same_meta(y->m, h->xm),
               "heff_apply: x/y do not have the operator's block list");
   hipStream_t s = stream();
   const size_t bytes = h->xm->total * sizeof(double);
@@ -285,7 +290,9 @@ int ttg_heff_info(ttg_heff h, double* flops, int64_t* ntasks3, double* stage_flo
 }
 
 int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms) {
-  TTG_REQUIRE(h && n > 0, "heff_time: bad argument");
+  TTG_REQUIRE(h && x && y && n > 0 && ms, "heff_time: bad argument");
+  TTG_REQUIRE(same_meta(x->m, h->xm) && same_meta(y->m, h->xm),
+              "heff_time: x/y do not have the operator's block list");
   hipStream_t s = stream();
   const size_t bytes = h->xm->total * sizeof(double);
   hipEvent_t e0, e1;
@@ -310,6 +317,7 @@ int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms) {
 
 int ttg_heff_stage_time(ttg_heff h, ttg_tensor x, int n, double* ms3) {
   TTG_REQUIRE(h && x && n > 0 && ms3, "heff_stage_time: bad argument");
+  TTG_REQUIRE(same_meta(x->m, h->xm), "heff_stage_time: x does not have the operator's block list");
   hipStream_t s = stream();
   const size_t bytes = h->xm->total * sizeof(double);
   TTG_HIP(hipMemcpyAsync(h->xbuf, x->d_data, bytes, hipMemcpyDeviceToDevice, s));

@@ -651,34 +651,27 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
 
   const int nA = A->nblocks, nB = B->nblocks;
   hipStream_t s = stream();
-  // scratch
+  // scratch (returned to the pool on every exit path)
+  Scratch sc;
   KeyArrays k{};
   const int nmax = std::max(1, std::max(nA, nB));
-  k.kUL = dalloc<uint64_t>(nmax); k.kSA = dalloc<uint64_t>(nmax); k.kUR = dalloc<uint64_t>(nmax);
-  k.kUM = dalloc<uint64_t>(nmax); k.kSB = dalloc<uint64_t>(nmax);
-  k.sUL = dalloc<uint64_t>(nmax); k.sSA = dalloc<uint64_t>(nmax); k.sUR = dalloc<uint64_t>(nmax);
-  k.sUM = dalloc<uint64_t>(nmax); k.sSB = dalloc<uint64_t>(nmax);
-  k.vSA = dalloc<int>(nmax); k.vSB = dalloc<int>(nmax); k.svSA = dalloc<int>(nmax); k.svSB = dalloc<int>(nmax);
-  uint64_t* uL = dalloc<uint64_t>(nmax);
-  uint64_t* uR = dalloc<uint64_t>(nmax);
-  uint64_t* uM = dalloc<uint64_t>(nmax);
+  k.kUL = sc.get<uint64_t>(nmax); k.kSA = sc.get<uint64_t>(nmax); k.kUR = sc.get<uint64_t>(nmax);
+  k.kUM = sc.get<uint64_t>(nmax); k.kSB = sc.get<uint64_t>(nmax);
+  k.sUL = sc.get<uint64_t>(nmax); k.sSA = sc.get<uint64_t>(nmax); k.sUR = sc.get<uint64_t>(nmax);
+  k.sUM = sc.get<uint64_t>(nmax); k.sSB = sc.get<uint64_t>(nmax);
+  k.vSA = sc.get<int>(nmax); k.vSB = sc.get<int>(nmax); k.svSA = sc.get<int>(nmax); k.svSB = sc.get<int>(nmax);
+  uint64_t* uL = sc.get<uint64_t>(nmax);
+  uint64_t* uR = sc.get<uint64_t>(nmax);
+  uint64_t* uM = sc.get<uint64_t>(nmax);
   QTab q{};
-  q.qL0 = dalloc<int>(nmax); q.qL1 = dalloc<int>(nmax); q.qR0 = dalloc<int>(nmax);
-  q.qR1 = dalloc<int>(nmax); q.qM0 = dalloc<int>(nmax); q.qM1 = dalloc<int>(nmax);
-  q.qbase = dalloc<int64_t>(nmax + 1);
-  PlanHdr* d_hdr = dalloc<PlanHdr>(1);
+  q.qL0 = sc.get<int>(nmax); q.qL1 = sc.get<int>(nmax); q.qR0 = sc.get<int>(nmax);
+  q.qR1 = sc.get<int>(nmax); q.qM0 = sc.get<int>(nmax); q.qM1 = sc.get<int>(nmax);
+  q.qbase = sc.get<int64_t>(nmax + 1);
+  PlanHdr* d_hdr = sc.get<PlanHdr>(1);
+  TTG_TRY(sc.ok("plan"));
   PlanHdr* h_hdr = pinned_hdr();
   TTG_REQUIRE(h_hdr, "plan: pinned host header allocation failed");
   TTG_HIP(hipMemsetAsync(d_hdr, 0, sizeof(PlanHdr), s));
-
-  auto release = [&]() {
-    dfree(k.kUL); dfree(k.kSA); dfree(k.kUR); dfree(k.kUM); dfree(k.kSB);
-    dfree(k.sUL); dfree(k.sSA); dfree(k.sUR); dfree(k.sUM); dfree(k.sSB);
-    dfree(k.vSA); dfree(k.vSB); dfree(k.svSA); dfree(k.svSB);
-    dfree(uL); dfree(uR); dfree(uM);
-    dfree(q.qL0); dfree(q.qL1); dfree(q.qR0); dfree(q.qR1); dfree(q.qM0); dfree(q.qM1);
-    dfree(q.qbase); dfree(d_hdr);
-  };
 
   hipLaunchKernelGGL(k_plan_keys, dim3((nmax + 255) / 256), dim3(256), 0, s, A->view(), B->view(),
                      ps, k);
@@ -694,26 +687,21 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   TTG_HIP(hipStreamSynchronize(s));
   const int64_t nblocksC = h_hdr->nblocksC;
   const int nQ = (int)h_hdr->nQ;
-  if (nblocksC >= (1 << 30)) {
-    release();
-    set_error("plan: too many result blocks");
-    return 1;
-  }
+  TTG_REQUIRE(nblocksC < (1 << 30), "plan: too many result blocks");
 
   // result structure
   std::vector<std::pair<const TMeta*, int>> cidx;
   for (int j = 0; j < ps.nfa; j++) cidx.push_back({A.get(), ps.fa[j]});
   for (int j = 0; j < ps.nfb; j++) cidx.push_back({B.get(), ps.fb[j]});
   MetaPtr C = meta_from_indices(cidx);
-  if (!C) { release(); return 1; }
-  C->nblocks = (int)nblocksC;
-  C->d_qi = dalloc<int>((size_t)nblocksC * std::max(C->rank, 1));
-  C->d_off = dalloc<int64_t>((size_t)nblocksC + 1);
+  if (!C) return 1;
+  TTG_TRY(meta_alloc_blocks(*C, (int)nblocksC, 0));
   BlkArrays ba{};
-  ba.size = dalloc<int64_t>(nblocksC + 1);
-  ba.ntask = dalloc<int64_t>(nblocksC + 1);
-  ba.flops = dalloc<double>(nblocksC + 1);
-  int64_t* task_off = dalloc<int64_t>(nblocksC + 1);
+  ba.size = sc.get<int64_t>(nblocksC + 1);
+  ba.ntask = sc.get<int64_t>(nblocksC + 1);
+  ba.flops = sc.get<double>(nblocksC + 1);
+  int64_t* task_off = sc.get<int64_t>(nblocksC + 1);
+  TTG_TRY(sc.ok("plan"));
   ba.c_qi = C->d_qi;
   EmitArrays ea{};
   const unsigned gb = (unsigned)std::max<int64_t>(1, (nblocksC + 255) / 256);
@@ -723,21 +711,12 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
                      ba.flops, C->d_off, task_off, d_hdr);
   TTG_HIP(hipMemcpyAsync(h_hdr, d_hdr, sizeof(PlanHdr), hipMemcpyDeviceToHost, s));
   TTG_HIP(hipStreamSynchronize(s));
-  if (h_hdr->err) {
-    dfree(ba.size); dfree(ba.ntask); dfree(ba.flops); dfree(task_off);
-    release();
-    set_error("plan: operand layout needs more than two strides per GEMM dimension");
-    return 1;
-  }
+  TTG_REQUIRE(!h_hdr->err, "plan: operand layout needs more than two strides per GEMM dimension");
   C->total = (size_t)h_hdr->totalC;
-  if (h_hdr->opmax >= (int64_t(1) << 28)) {   // the GEMM's 32-bit byte offsets per block
-    dfree(ba.size); dfree(ba.ntask); dfree(ba.flops); dfree(task_off);
-    release();
-    set_error("plan: an operand block exceeds 2^28 elements");
-    return 1;
-  }
+  // the GEMM's 32-bit byte offsets per block
+  TTG_REQUIRE(h_hdr->opmax < (int64_t(1) << 28), "plan: an operand block exceeds 2^28 elements");
 
-  ttg_plan p = new ttg_plan_s();
+  std::unique_ptr<ttg_plan_s> p(new ttg_plan_s());
   p->A = A;
   p->B = B;
   p->C = C;
@@ -761,6 +740,10 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   p->d_tasks = dalloc<GemmTask>(nt);
   p->d_cblk = dalloc<CBlock>(std::max<int64_t>(1, nblocksC));
   p->d_tuples = dalloc<int>(6 * nt);
+  if (!p->d_tasks || !p->d_cblk || !p->d_tuples) {
+    set_error(std::string("plan: ") + ttg_last_error());
+    return 1;
+  }
   ba.task_off = task_off;
   ba.c_off = C->d_off;
   ea.tasks = p->d_tasks;
@@ -772,13 +755,8 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   if (nblocksC)
     TTG_HIP(hipMemcpyAsync(hcb.data(), p->d_cblk, sizeof(CBlock) * nblocksC, hipMemcpyDeviceToHost, s));
   TTG_HIP(hipStreamSynchronize(s));
-  dfree(ba.size); dfree(ba.ntask); dfree(ba.flops); dfree(task_off);
-  release();
-  if (build_tiles(*p, hcb)) {
-    delete p;
-    return 1;
-  }
-  *out = p;
+  TTG_TRY(build_tiles(*p, hcb));
+  *out = p.release();
   return 0;
 }
 
@@ -819,9 +797,9 @@ int ttg_plan_new_output(ttg_plan p, ttg_tensor* C) {
 
 int ttg_contract_exec(ttg_plan p, ttg_tensor A, ttg_tensor B, ttg_tensor C) {
   TTG_REQUIRE(p && A && B && C, "contract_exec: null argument");
-  TTG_REQUIRE((A->m == p->A || same_structure(A, A)) && A->m->nblocks == p->A->nblocks &&
-                  B->m->nblocks == p->B->nblocks && C->m->nblocks == p->C->nblocks,
-              "contract_exec: tensors do not match the plan");
+  // the plan's task offsets are only valid for the block lists it was built on
+  TTG_REQUIRE(same_meta(A->m, p->A) && same_meta(B->m, p->B) && same_meta(C->m, p->C),
+              "contract_exec: tensors do not have the block lists the plan was built on");
   if (p->prec == TTG_FP64) {
     TTG_TRY(gemm_launch(*p, A->d_data, B->d_data, C->d_data, true, stream()));
   } else {

@@ -96,6 +96,12 @@ void dfree(void* p) {
   g_pool.live.erase(it);
 }
 
+int Scratch::ok(const char* what) const {
+  if (!failed_) return 0;
+  set_error(std::string(what) + ": " + g_error);   // g_error holds the pool's reason
+  return 1;
+}
+
 int ensure_init() {
   if (g_rt.inited) return 0;
   return ttg_init(0);

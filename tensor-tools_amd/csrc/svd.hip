@@ -360,8 +360,12 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
       }
       __syncthreads();
     }
+    // every thread reads the flag before thread 0 may clear it for the next
+    // sweep: the barrier between the read and the reset keeps the decision
+    // uniform over the workgroup
+    const int rotated = any_rot;
     __syncthreads();
-    if (!any_rot) break;
+    if (!rotated) break;
   }
   for (int i = threadIdx.x; i < n; i += blockDim.x) evals[eoff[q] + i] = g[pk(i, i)];
   if (lay.where & 2)
@@ -776,23 +780,17 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
     for (int rc = 0; rc < N / kSub; rc++)
       for (int b = 0; b < npair; b++) titems.push_back(int4{li, rc, b, 1});
   }
-  BJSector* d_sec = dalloc<BJSector>(ns);
-  int2* d_pit = dalloc<int2>(pitems.size());
-  int4* d_tit = dalloc<int4>(titems.size());
-  double* Gp = dalloc<double>(gp);
-  double* Vp = dalloc<double>(gp);
-  double* Qp = dalloc<double>(qp);
-  int* act = dalloc<int>(ns);
-  int* flag = dalloc<int>(ns);
-  int* rot = dalloc<int>(pitems.size());
-  auto release = [&]() {
-    dfree(d_sec); dfree(d_pit); dfree(d_tit); dfree(Gp); dfree(Vp); dfree(Qp); dfree(act); dfree(flag); dfree(rot);
-  };
-  if (!d_sec || !d_pit || !d_tit || !Gp || !Vp || !Qp || !act || !flag || !rot) {
-    release();
-    set_error("svd: block Jacobi allocation failed");
-    return 1;
-  }
+  Scratch sc;
+  BJSector* d_sec = sc.get<BJSector>(ns);
+  int2* d_pit = sc.get<int2>(pitems.size());
+  int4* d_tit = sc.get<int4>(titems.size());
+  double* Gp = sc.get<double>(gp);
+  double* Vp = sc.get<double>(gp);
+  double* Qp = sc.get<double>(qp);
+  int* act = sc.get<int>(ns);
+  int* flag = sc.get<int>(ns);
+  int* rot = sc.get<int>(pitems.size());
+  TTG_TRY(sc.ok("svd: block Jacobi"));
   std::vector<int> hact(ns, 1);
   TTG_HIP(hipMemcpyAsync(d_sec, sec.data(), sizeof(BJSector) * ns, hipMemcpyHostToDevice, s));
   TTG_HIP(hipMemcpyAsync(d_pit, pitems.data(), sizeof(int2) * pitems.size(), hipMemcpyHostToDevice, s));
@@ -823,7 +821,6 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   hipLaunchKernelGGL(k_bjac_finish, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, d_eoff, Gp, Vp,
                      Vev, ev);
   TTG_HIP(hipStreamSynchronize(s));
-  release();
   return 0;
 }
 
@@ -1276,6 +1273,7 @@ static int run_host_gemm(HostGemm& hg, const double* A, const double* B, double*
   p.b_kcontig = g0.sbk0 == 1;
   p.d_tasks = dalloc<GemmTask>(hg.tasks.size());
   p.d_cblk = dalloc<CBlock>(hg.cblk.size());
+  TTG_REQUIRE(p.d_tasks && p.d_cblk, (std::string("svd: ") + ttg_last_error()).c_str());
   hipStream_t s = stream();
   TTG_HIP(hipMemcpyAsync(p.d_tasks, hg.tasks.data(), sizeof(GemmTask) * hg.tasks.size(),
                          hipMemcpyHostToDevice, s));
@@ -1300,12 +1298,12 @@ static GemmTask gtask(int64_t a_off, int64_t b_off, int em0, int64_t sam0, int e
 
 // device copy of a host vector (nullptr, with the error set, on failure)
 template <class T>
-static T* up(const std::vector<T>& v) {
-  T* d = dalloc<T>(std::max<size_t>(v.size(), 1));
+static T* up(Scratch& sc, const std::vector<T>& v) {
+  T* d = sc.get<T>(std::max<size_t>(v.size(), 1));
   if (d && !v.empty() &&
       hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream()) != hipSuccess) {
-    set_error("svd: host-to-device copy failed");
-    dfree(d);
+    set_error("host-to-device copy failed");
+    sc.hold(nullptr);   // marks the group failed
     return nullptr;
   }
   return d;
@@ -1329,16 +1327,14 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
   const int nb = std::max(1, std::min(16, (int)(163840 / (8 * (int64_t)nmax)) - 2));
   const size_t lds = sizeof(double) * ((size_t)nb * nmax + nmax) + sizeof(int) * nmax;
   TTG_REQUIRE(lds <= 163840, "svd: sector too large for the pivoted Cholesky panel");
-  std::vector<void*> tmp;
-  auto release = [&]() { for (void* p : tmp) dfree(p); };
-  PcSector* d_ps = dalloc<PcSector>(ns);
-  double* R = dalloc<double>(gtotal);
-  double* dg = dalloc<double>(dtot);
-  int* pg = dalloc<int>(dtot);
-  int* rank = dalloc<int>(ns);
-  double* dmax = dalloc<double>(ns);
-  for (void* p : std::initializer_list<void*>{d_ps, R, dg, pg, rank, dmax}) tmp.push_back(p);
-  if (!d_ps || !R || !dg || !pg || !rank || !dmax) { release(); set_error("svd: allocation failed"); return 1; }
+  Scratch sc;
+  PcSector* d_ps = sc.get<PcSector>(ns);
+  double* R = sc.get<double>(gtotal);
+  double* dg = sc.get<double>(dtot);
+  int* pg = sc.get<int>(dtot);
+  int* rank = sc.get<int>(ns);
+  double* dmax = sc.get<double>(ns);
+  TTG_TRY(sc.ok("svd: LR step"));
   TTG_HIP(hipMemcpyAsync(d_ps, ps.data(), sizeof(PcSector) * ns, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_pchol_init, dim3(ns), dim3(kPcThreads), 0, s, d_ps, G, dg, pg, rank, dmax);
   TTG_HIP(hipFuncSetAttribute((const void*)k_pchol_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1367,20 +1363,15 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
     if (r > kJacBlockMin) big2.push_back(li);
     else if (r > 0) { any_small = true; lds_need = std::max(lds_need, jac_layout(r).total); }
   }
-  double* G2 = dalloc<double>(std::max<int64_t>(1, goff2[ns]));
-  double* W2 = dalloc<double>(std::max<int64_t>(1, goff2[ns]));
-  double* gs2 = dalloc<double>(std::max<int64_t>(1, poff2[ns]));
-  double* ev2 = dalloc<double>(std::max<int64_t>(1, eoff2[ns]));
-  int* d_r2 = up(r2);
-  int64_t* d_goff2 = up(goff2);
-  int64_t* d_poff2 = up(poff2);
-  int64_t* d_eoff2 = up(eoff2);
-  for (void* p : std::initializer_list<void*>{G2, W2, gs2, ev2, d_r2, d_goff2, d_poff2, d_eoff2}) tmp.push_back(p);
-  if (!G2 || !W2 || !gs2 || !ev2 || !d_r2 || !d_goff2 || !d_poff2 || !d_eoff2) {
-    release();
-    set_error("svd: allocation failed");
-    return 1;
-  }
+  double* G2 = sc.get<double>(std::max<int64_t>(1, goff2[ns]));
+  double* W2 = sc.get<double>(std::max<int64_t>(1, goff2[ns]));
+  double* gs2 = sc.get<double>(std::max<int64_t>(1, poff2[ns]));
+  double* ev2 = sc.get<double>(std::max<int64_t>(1, eoff2[ns]));
+  int* d_r2 = up(sc, r2);
+  int64_t* d_goff2 = up(sc, goff2);
+  int64_t* d_poff2 = up(sc, poff2);
+  int64_t* d_eoff2 = up(sc, eoff2);
+  TTG_TRY(sc.ok("svd: LR step"));
   // G' = R R^T: A(k, j) = R[k n + j], B(j, k') = R[k' n + j]
   {
     HostGemm hg;
@@ -1391,7 +1382,7 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
       const int t = (int)hg.tasks.size() - 1;
       hg.cblk.push_back(CBlock{goff2[li], r, r, t, t + 1});
     }
-    if (run_host_gemm(hg, R, R, G2)) { release(); return 1; }
+    TTG_TRY(run_host_gemm(hg, R, R, G2));
   }
   // eigenpairs of G' (relative convergence test only)
   if (any_small) {
@@ -1402,10 +1393,7 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
                        d_eoff2, 40, kJacBlockMin, 0.0);
     TTG_HIP(hipGetLastError());
   }
-  if (!big2.empty() && block_jacobi(big2, r2, d_goff2, d_eoff2, G2, W2, ev2, s, 40, 0.0)) {
-    release();
-    return 1;
-  }
+  if (!big2.empty()) TTG_TRY(block_jacobi(big2, r2, d_goff2, d_eoff2, G2, W2, ev2, s, 40, 0.0));
   // V = R^T W Lambda^(-1/2): A(j, k) = R[k n + j], B(k, c) = W[k + r c]
   {
     HostGemm hv;
@@ -1416,11 +1404,10 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
       const int t = (int)hv.tasks.size() - 1;
       hv.cblk.push_back(CBlock{ps[li].goff, n, r, t, t + 1});
     }
-    if (run_host_gemm(hv, R, W2, Vev)) { release(); return 1; }
+    TTG_TRY(run_host_gemm(hv, R, W2, Vev));
   }
   hipLaunchKernelGGL(k_lr_finish, dim3(nmax, ns), dim3(256), 0, s, d_ps, d_r2, d_eoff2, ev2, d_eoff, Vev, ev);
   TTG_HIP(hipStreamSynchronize(s));
-  release();
   return 0;
 }
 
@@ -1442,20 +1429,19 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   const int nb = m.nblocks;
   TTG_REQUIRE(nb > 0, "svd: tensor without blocks");
   // ---- 1. bookkeeping
-  uint64_t *kl = dalloc<uint64_t>(nb), *kr = dalloc<uint64_t>(nb);
-  uint64_t *sl = dalloc<uint64_t>(nb), *sr = dalloc<uint64_t>(nb);
-  int *ids = dalloc<int>(nb), *sid = dalloc<int>(nb);
+  Scratch sc;   // every buffer below goes back to the pool on return
+  uint64_t *kl = sc.get<uint64_t>(nb), *kr = sc.get<uint64_t>(nb);
+  uint64_t *sl = sc.get<uint64_t>(nb), *sr = sc.get<uint64_t>(nb);
+  int *ids = sc.get<int>(nb), *sid = sc.get<int>(nb);
   SvdTab t{};
-  t.uL = dalloc<uint64_t>(nb); t.uR = dalloc<uint64_t>(nb);
-  t.lsz = dalloc<int64_t>(nb); t.loff = dalloc<int64_t>(nb);
-  t.rsz = dalloc<int64_t>(nb); t.roff = dalloc<int64_t>(nb);
-  t.qL0 = dalloc<int>(nb); t.qL1 = dalloc<int>(nb); t.qR0 = dalloc<int>(nb); t.qR1 = dalloc<int>(nb);
-  t.rows = dalloc<int64_t>(nb); t.cols = dalloc<int64_t>(nb);
-  t.qc = dalloc<uint32_t>(nb);
-  t.hdr = dalloc<int64_t>(4);
-  std::vector<void*> scratch{kl, kr, sl, sr, ids, sid, t.uL, t.uR, t.lsz, t.loff, t.rsz, t.roff,
-                             t.qL0, t.qL1, t.qR0, t.qR1, t.rows, t.cols, t.qc, t.hdr};
-  auto release = [&]() { for (void* p : scratch) dfree(p); };
+  t.uL = sc.get<uint64_t>(nb); t.uR = sc.get<uint64_t>(nb);
+  t.lsz = sc.get<int64_t>(nb); t.loff = sc.get<int64_t>(nb);
+  t.rsz = sc.get<int64_t>(nb); t.roff = sc.get<int64_t>(nb);
+  t.qL0 = sc.get<int>(nb); t.qL1 = sc.get<int>(nb); t.qR0 = sc.get<int>(nb); t.qR1 = sc.get<int>(nb);
+  t.rows = sc.get<int64_t>(nb); t.cols = sc.get<int64_t>(nb);
+  t.qc = sc.get<uint32_t>(nb);
+  t.hdr = sc.get<int64_t>(4);
+  TTG_TRY(sc.ok("svd"));
   const unsigned g1 = (nb + 255) / 256;
   hipLaunchKernelGGL(k_svd_keys, dim3(g1), dim3(256), 0, s, m.view(), nleft, kl, kr, ids);
   hipLaunchKernelGGL(k_sort_keys, dim3(g1), dim3(256), 0, s, kl, ids, nb, sl, sid);
@@ -1497,37 +1483,27 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     soff[q + 1] = soff[q] + kq[q];
     selo[q + 1] = selo[q] + (int64_t)nq[q] * kq[q];
   }
-  double* mat = dalloc<double>(moff[nQ]);
-  double* G = dalloc<double>(goff[nQ]);
-  double* Vev = dalloc<double>(goff[nQ]);
-  double* gs = dalloc<double>(poff[nQ]);
-  double* ev = dalloc<double>(eoff[nQ]);
-  double* Esel = dalloc<double>(selo[nQ]);
-  double* svals = dalloc<double>(soff[nQ]);
-  int* ssec = dalloc<int>(soff[nQ]);
-  int* order = dalloc<int>(soff[nQ]);
-  double* ssorted = dalloc<double>(soff[nQ]);
-  int* newdim = dalloc<int>(nQ);
-  double* tout = dalloc<double>(2);
-  int64_t* d_moff = up(moff);
-  int64_t* d_goff = up(goff);
-  int64_t* d_poff = up(poff);
-  int64_t* d_eoff = up(eoff);
-  int64_t* d_soff = up(soff);
-  int64_t* d_selo = up(selo);
-  int* d_nq = up(nq);
-  int* d_kq = up(kq);
-  bool alloc_ok = true;
-  for (void* p : std::initializer_list<void*>{mat, G, Vev, gs, ev, Esel, svals, ssec, order, ssorted, newdim, tout,
-                                              d_moff, d_goff, d_poff, d_eoff, d_soff, d_selo, d_nq, d_kq}) {
-    scratch.push_back(p);
-    alloc_ok = alloc_ok && p;
-  }
-  if (!alloc_ok) {
-    release();
-    set_error("svd: device allocation failed");
-    return 1;
-  }
+  double* mat = sc.get<double>(moff[nQ]);
+  double* G = sc.get<double>(goff[nQ]);
+  double* Vev = sc.get<double>(goff[nQ]);
+  double* gs = sc.get<double>(poff[nQ]);
+  double* ev = sc.get<double>(eoff[nQ]);
+  double* Esel = sc.get<double>(selo[nQ]);
+  double* svals = sc.get<double>(soff[nQ]);
+  int* ssec = sc.get<int>(soff[nQ]);
+  int* order = sc.get<int>(soff[nQ]);
+  double* ssorted = sc.get<double>(soff[nQ]);
+  int* newdim = sc.get<int>(nQ);
+  double* tout = sc.get<double>(2);
+  int64_t* d_moff = up(sc, moff);
+  int64_t* d_goff = up(sc, goff);
+  int64_t* d_poff = up(sc, poff);
+  int64_t* d_eoff = up(sc, eoff);
+  int64_t* d_soff = up(sc, soff);
+  int64_t* d_selo = up(sc, selo);
+  int* d_nq = up(sc, nq);
+  int* d_kq = up(sc, kq);
+  TTG_TRY(sc.ok("svd"));
   // ---- 2. gather
   fill_f64(mat, moff[nQ], 0.0);
   hipLaunchKernelGGL(k_svd_gather, dim3(std::min(nb, 4096)), dim3(256), 0, s, m.view(), nleft,
@@ -1544,24 +1520,19 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
         hg.tasks.push_back(gtask(moff[q], moff[q], (int)c, r, (int)r, 1, 1, (int)c, r, (int)r));
       hg.cblk.push_back(CBlock{goff[q], n, n, q, q + 1});
     }
-    if (run_host_gemm(hg, mat, mat, G)) { release(); return 1; }
+    TTG_TRY(run_host_gemm(hg, mat, mat, G));
   }
   // a sector whose Gram matrix overflowed (entries of A above ~1e154) or that
   // holds a NaN has no meaningful eigenpairs: fail before the eigensolver
   {
-    int* bad = dalloc<int>(1);
-    if (!bad) { release(); set_error("svd: device allocation failed"); return 1; }
-    scratch.push_back(bad);
+    int* bad = sc.get<int>(1);
+    TTG_TRY(sc.ok("svd"));
     int hbad = 0;
     TTG_HIP(hipMemcpyAsync(bad, &hbad, sizeof(int), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_svd_finite, dim3(nQ), dim3(256), 0, s, nQ, d_nq, d_goff, G, bad);
     TTG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     TTG_HIP(hipStreamSynchronize(s));
-    if (hbad) {
-      release();
-      set_error("svd: a sector's Gram matrix is not finite (tensor entries above ~1e154, or NaN)");
-      return 1;
-    }
+    TTG_REQUIRE(!hbad, "svd: a sector's Gram matrix is not finite (tensor entries above ~1e154, or NaN)");
   }
   // ---- 4. eigensolver
   std::vector<int> big;
@@ -1584,12 +1555,9 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   }
   // large sectors: the LR step when the truncation drops zero singular
   // values (cutoff > 0), else block Jacobi on G
-  if (!big.empty() &&
-      (cutoff > 0 ? lr_eigen(big, nq, goff, eoff, d_eoff, goff[nQ], G, Vev, ev, s)
-                  : block_jacobi(big, nq, d_goff, d_eoff, G, Vev, ev, s, 40, kJacFloor))) {
-    release();
-    return 1;
-  }
+  if (!big.empty())
+    TTG_TRY(cutoff > 0 ? lr_eigen(big, nq, goff, eoff, d_eoff, goff[nQ], G, Vev, ev, s)
+                       : block_jacobi(big, nq, d_goff, d_eoff, G, Vev, ev, s, 40, kJacFloor));
   // ---- 5. selection and truncation
   hipLaunchKernelGGL(k_svd_select, dim3(nQ), dim3(256), 0, s, d_nq, d_kq, d_eoff, ev, d_goff, Vev,
                      d_soff, svals, ssec, d_selo, Esel);
@@ -1662,17 +1630,20 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     TTG_HIP(hipMemcpyAsync(mid->d_off, soffv.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, s));
     TTG_HIP(hipStreamSynchronize(s));
   }
-  int* d_mid = up(midnum);
-  int* d_ub0 = up(ublk0);
-  int* d_vb0 = up(vblk0);
-  int64_t* d_sout = up(sout);
-  scratch.push_back(d_mid); scratch.push_back(d_ub0); scratch.push_back(d_vb0); scratch.push_back(d_sout);
+  int* d_mid = up(sc, midnum);
+  int* d_ub0 = up(sc, ublk0);
+  int* d_vb0 = up(sc, vblk0);
+  int64_t* d_sout = up(sc, sout);
+  TTG_TRY(sc.ok("svd"));
   hipLaunchKernelGGL(k_svd_emit, dim3(1), dim3(1024), 0, s, m.view(), nleft, t, nQ, newdim, d_mid,
                      Um->d_qi, Um->d_off, Vm->d_qi, Vm->d_off);
   ttg_tensor U, V, Sv;
   TTG_TRY(tensor_from_meta(Um, false, &U));
+  std::unique_ptr<ttg_tensor_s> Uown(U);
   TTG_TRY(tensor_from_meta(Vm, false, &V));
+  std::unique_ptr<ttg_tensor_s> Vown(V);
   TTG_TRY(tensor_from_meta(mid, false, &Sv));
+  std::unique_ptr<ttg_tensor_s> Sown(Sv);
   hipLaunchKernelGGL(k_svd_svals, dim3(nQ), dim3(64), 0, s, nQ, newdim, d_soff, svals, d_sout, Sv->d_data);
   // absorbed factor: MoveFromLeft  W = E^T A (k x cols), U = E (rows x k)
   //                  MoveFromRight U = A E (rows x k),   W = E^T (k x cols)
@@ -1681,11 +1652,11 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     umo[q + 1] = umo[q] + rows[q] * nd[q];
     wmo[q + 1] = wmo[q] + (int64_t)nd[q] * cols[q];
   }
-  double* Umat = dalloc<double>(umo[nQ]);
-  double* Wmat = dalloc<double>(wmo[nQ]);
-  int64_t* d_umo = up(umo);
-  int64_t* d_wmo = up(wmo);
-  scratch.push_back(Umat); scratch.push_back(Wmat); scratch.push_back(d_umo); scratch.push_back(d_wmo);
+  double* Umat = sc.get<double>(umo[nQ]);
+  double* Wmat = sc.get<double>(wmo[nQ]);
+  int64_t* d_umo = up(sc, umo);
+  int64_t* d_wmo = up(sc, wmo);
+  TTG_TRY(sc.ok("svd"));
   {
     HostGemm hg;
     for (int q = 0; q < nQ; q++) {
@@ -1700,8 +1671,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
       hg.cblk.push_back(fromLeft ? CBlock{wmo[q], k, (int)c, tix, tix + 1}
                                  : CBlock{umo[q], (int)r, k, tix, tix + 1});
     }
-    int rc = fromLeft ? run_host_gemm(hg, Esel, mat, Wmat) : run_host_gemm(hg, mat, Esel, Umat);
-    if (rc) { release(); return 1; }
+    TTG_TRY(fromLeft ? run_host_gemm(hg, Esel, mat, Wmat) : run_host_gemm(hg, mat, Esel, Umat));
   }
   if (fromLeft) {   // U_q = E restricted to k columns: Esel already (n = rows) x k, ld rows
     for (int q = 0; q < nQ; q++)
@@ -1715,9 +1685,8 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   hipLaunchKernelGGL(k_svd_scatter, dim3(nQ), dim3(256), 0, s, t, nQ, newdim, d_ub0, d_vb0, Um->d_off,
                      Vm->d_off, Umat, d_umo, Wmat, d_wmo, U->d_data, V->d_data);
   TTG_HIP(hipStreamSynchronize(s));
-  release();
-  *Uout = U;
-  *Vout = V;
-  *Sout = Sv;
+  *Uout = Uown.release();
+  *Vout = Vown.release();
+  *Sout = Sown.release();
   return 0;
 }

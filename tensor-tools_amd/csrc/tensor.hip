@@ -135,9 +135,12 @@ int tensor_from_meta(const MetaPtr& m, bool zero, ttg_tensor* out) {
   return 0;
 }
 
-bool same_structure(ttg_tensor a, ttg_tensor b) {
-  if (a->m == b->m) return true;
-  const TMeta &x = *a->m, &y = *b->m;
+bool same_structure(ttg_tensor a, ttg_tensor b) { return same_meta(a->m, b->m); }
+
+bool same_meta(const MetaPtr& a, const MetaPtr& b) {
+  if (a == b) return true;
+  if (!a || !b) return false;
+  const TMeta &x = *a, &y = *b;
   if (x.nblocks != y.nblocks || x.total != y.total || !x.same_indices(y)) return false;
   // compare block lists on the host (small)
   std::vector<int> qa((size_t)x.nblocks * x.rank), qb(qa.size());
@@ -350,7 +353,9 @@ int ttg_tensor_init_blocks(int rank, const int* arrows, const int* nsec, const i
     ncomb *= nsec[j];
     TTG_REQUIRE(ncomb < ((int64_t)1 << 40), "initBlock: too many sector combinations");
   }
-  int64_t* hdr = dalloc<int64_t>(2);
+  Scratch sc;
+  int64_t* hdr = sc.get<int64_t>(2);
+  TTG_TRY(sc.ok("initBlock"));
   int64_t h[2];
   hipLaunchKernelGGL(k_init_blocks, dim3(1), dim3(1024), 0, stream(), m->view(), ncomb, 0,
                      (int*)nullptr, (int64_t*)nullptr, hdr);
@@ -360,7 +365,6 @@ int ttg_tensor_init_blocks(int rank, const int* arrows, const int* nsec, const i
   TTG_TRY(meta_alloc_blocks(*m, (int)h[0], (size_t)h[1]));
   hipLaunchKernelGGL(k_init_blocks, dim3(1), dim3(1024), 0, stream(), m->view(), ncomb, 1,
                      m->d_qi, m->d_off, hdr);
-  dfree(hdr);
   return tensor_from_meta(m, true, out);
 }
 
@@ -485,6 +489,7 @@ int ttg_dot(ttg_tensor x, ttg_tensor y, double* out) {
   TTG_REQUIRE(x && y, "null tensor");
   TTG_REQUIRE(same_structure(x, y), "dot: tensors have different block lists");
   if (!g_scalar) g_scalar = dalloc<double>(1);
+  TTG_REQUIRE(g_scalar, "scalar allocation failed");
   TTG_TRY(dev_dot(x->d_data, y->d_data, x->m->total, g_scalar));
   return read_scalar(out);
 }
@@ -492,6 +497,7 @@ int ttg_dot(ttg_tensor x, ttg_tensor y, double* out) {
 int ttg_nrm2(ttg_tensor x, double* out) {
   TTG_REQUIRE(x, "null tensor");
   if (!g_scalar) g_scalar = dalloc<double>(1);
+  TTG_REQUIRE(g_scalar, "scalar allocation failed");
   TTG_TRY(dev_dot(x->d_data, x->d_data, x->m->total, g_scalar));
   double s;
   TTG_TRY(read_scalar(&s));

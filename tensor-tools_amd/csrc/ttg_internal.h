@@ -54,6 +54,39 @@ void dfree(void* p);
 template <class T>
 T* dalloc(size_t n) { return (T*)dmalloc(n * sizeof(T) + (n == 0 ? 8 : 0)); }
 
+// Scratch buffers of one library call.  Every buffer goes back to the pool
+// when the object leaves scope, on the error returns too, and a refused
+// allocation is remembered: the caller checks ok() after a group of
+// allocations and fails with the pool's reason before it launches anything
+// on a null pointer.
+class Scratch {
+ public:
+  Scratch() = default;
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  ~Scratch() {
+    for (void* p : bufs_) dfree(p);
+  }
+  template <class T>
+  T* get(size_t n) {
+    T* p = dalloc<T>(n);
+    hold(p);
+    return p;
+  }
+  // take over a buffer allocated elsewhere (null marks a failure)
+  void hold(void* p) {
+    if (p) bufs_.push_back(p);
+    else failed_ = true;
+  }
+  bool failed() const { return failed_; }
+  // 0, or 1 with "<what>: <the pool's reason>" as the error
+  int ok(const char* what) const;
+
+ private:
+  std::vector<void*> bufs_;
+  bool failed_ = false;
+};
+
 // ---------------------------------------------------------------- tensors
 // Kernel-side view of a tensor's block structure.  Index sectors are stored
 // as (q0, q1, dim) int triples, all indices concatenated; idx_off[j] is the
@@ -115,6 +148,9 @@ namespace ttg {
 int tensor_from_meta(const MetaPtr& m, bool zero, ttg_tensor* out);
 inline void mark_written(ttg_tensor t) { t->version++; }
 int tensor_f32(ttg_tensor t, const float** out);
+// same index sectors and block list (hence the same packed offsets); equal
+// pointers short-cut, otherwise the block lists are compared on the host
+bool same_meta(const MetaPtr& a, const MetaPtr& b);
 bool same_structure(ttg_tensor a, ttg_tensor b);
 int block_scan_sum(const int64_t* in, int64_t* out_excl, int n);
 

@@ -37,15 +37,19 @@ using Q = qtensor<double>;
 std::string g_err;
 }  // namespace
 
-// every entry point converts a ttgpu::Error (or any std::exception) into an
-// error return: nullptr / NaN / -1, with the message in tth_last_error()
-#define TTH_GUARD(err, ...)                 \
-  try {                                     \
-    g_err.clear();                          \
-    __VA_ARGS__                             \
-  } catch (std::exception & e) {            \
-    g_err = e.what();                       \
-    return err;                             \
+// every entry point converts a ttgpu::Error (or any other exception) into an
+// error return: nullptr / NaN / -1, with the message in tth_last_error();
+// nothing thrown below crosses the C boundary
+#define TTH_GUARD(err, ...)                          \
+  try {                                              \
+    g_err.clear();                                   \
+    __VA_ARGS__                                      \
+  } catch (std::exception & e) {                     \
+    g_err = e.what();                                \
+    return err;                                      \
+  } catch (...) {                                    \
+    g_err = "unknown exception in the host layer";   \
+    return err;                                      \
   }
 static const double kNaN = std::numeric_limits<double>::quiet_NaN();
 
@@ -89,16 +93,20 @@ void* tth_init_block(int rank, const int* arrows, const int* levels, const int* 
 }
 
 // out: [rank, nsec_total, nblocks, names_len]
-void tth_tensor_info(void* h, int* out, long* total) {
+int tth_tensor_info(void* h, int* out, long* total) {
+  TTH_GUARD(-1,
   Q* t = (Q*)h;
   int ns = 0, nl = 0;
   for (auto& ix : t->idx_set) { ns += (int)ix.size(); nl += (int)ix.name().size() + 1; }
   out[0] = (int)t->rank; out[1] = ns; out[2] = t->nblocks(); out[3] = nl;
   *total = (long)t->total_size();
+  return 0;
+  )
 }
 
-void tth_tensor_export(void* h, int* arrows, int* levels, int* types, char* names, int* nsec,
-                       int* qn, int* qdim, int* block_qi, long* block_off, double* data) {
+int tth_tensor_export(void* h, int* arrows, int* levels, int* types, char* names, int* nsec,
+                      int* qn, int* qdim, int* block_qi, long* block_off, double* data) {
+  TTH_GUARD(-1,
   Q* t = (Q*)h;
   std::string nm;
   int off = 0;
@@ -125,6 +133,8 @@ void tth_tensor_export(void* h, int* arrows, int* levels, int* types, char* name
     auto d = t->to_host();
     memcpy(data, d.data(), d.size() * sizeof(double));
   }
+  return 0;
+  )
 }
 
 void tth_tensor_free(void* h) { delete (Q*)h; }
@@ -139,17 +149,19 @@ int tth_dag(void* A) { TTH_GUARD(-1, ((Q*)A)->dag(); return 0;) }
 
 // ---- big_qtensor ----------------------------------------------------------
 void* tth_big_new(void* L, void* W1, void* W2, void* R) {
-  auto* b = new big_qtensor<double>();
-  b->setLeft((Q*)L);
-  if (W1) b->addMid((Q*)W1);
-  if (W2) b->addMid((Q*)W2);
-  b->setRight((Q*)R);
-  return b;
+  TTH_GUARD(nullptr,
+    std::unique_ptr<big_qtensor<double>> b(new big_qtensor<double>());
+    b->setLeft((Q*)L);
+    if (W1) b->addMid((Q*)W1);
+    if (W2) b->addMid((Q*)W2);
+    b->setRight((Q*)R);
+    return b.release();
+  )
 }
 void tth_big_free(void* b) { delete (big_qtensor<double>*)b; }
 void* tth_big_product(void* b, void* x) { TTH_GUARD(nullptr, return new Q(((big_qtensor<double>*)b)->product(*(Q*)x));) }
 double tth_big_expec(void* b, void* x) { TTH_GUARD(kNaN, return ((big_qtensor<double>*)b)->expec(*(Q*)x);) }
-long tth_big_size(void* b) { return (long)((big_qtensor<double>*)b)->size(); }
+long tth_big_size(void* b) { TTH_GUARD(-1, return (long)((big_qtensor<double>*)b)->size();) }
 // the device operator (plans it for x's indices first)
 void* tth_big_heff(void* b, void* x) {
   TTH_GUARD(nullptr,
@@ -233,6 +245,7 @@ void* tth_mpo_autompo(int N, int id, int nt, const double* coef, const int* o1, 
 }
 int tth_heisenberg_terms(int kind, int Nx, int Ny, int yper, double J1, double J2, int cap,
                          double* coef, int* o1, int* i1, int* o2, int* i2) {
+  TTH_GUARD(-1,
   std::vector<int> type;
   auto b = square_lattice(Nx, Ny, yper != 0, &type, kind == 1);
   std::vector<double> J;
@@ -248,6 +261,7 @@ int tth_heisenberg_terms(int kind, int Nx, int Ny, int yper, double J1, double J
     n++;
   }
   return (int)t.size();
+  )
 }
 void* tth_mps_product(int N, int id, const int* state) {
   TTH_GUARD(nullptr,

@@ -27,8 +27,8 @@ def _lib():
             "tth_last_error": ([], C.c_char_p),
             "tth_tensor_import": ([_i, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip, _i, _ip, _dp], _p),
             "tth_init_block": ([_i, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip], _p),
-            "tth_tensor_info": ([_p, _ip, C.POINTER(_l)], None),
-            "tth_tensor_export": ([_p, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip, _ip, _lp, _dp], None),
+            "tth_tensor_info": ([_p, _ip, C.POINTER(_l)], _i),
+            "tth_tensor_export": ([_p, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip, _ip, _lp, _dp], _i),
             "tth_tensor_free": ([_p], None), "tth_tensor_copy": ([_p], _p),
             "tth_contract": ([_p, _p], _p), "tth_inner": ([_p, _p], _d), "tth_norm": ([_p], _d),
             "tth_add": ([_p, _p, _d], _i), "tth_scale": ([_p, _d], _i),
@@ -109,7 +109,7 @@ class PTensor:
     def to_flat(self) -> FlatTensor:
         info = np.zeros(4, np.int32)
         tot = _l(0)
-        _lib().tth_tensor_info(self.h, info, C.byref(tot))
+        _ok(_lib().tth_tensor_info(self.h, info, C.byref(tot)), "tensor info")
         rank, nst, nb, nl = (int(v) for v in info)
         arrows, levels, types, nsec = (np.zeros(max(rank, 1), np.int32) for _ in range(4))
         names = C.create_string_buffer(nl + 1)
@@ -118,7 +118,8 @@ class PTensor:
         bqi = np.zeros(max(nb * rank, 1), np.int32)
         boff = np.zeros(nb + 1, np.int64)
         data = np.zeros(max(tot.value, 1))
-        _lib().tth_tensor_export(self.h, arrows, levels, types, names, nsec, qn, qd, bqi, boff, data)
+        _ok(_lib().tth_tensor_export(self.h, arrows, levels, types, names, nsec, qn, qd, bqi, boff, data),
+            "tensor export")
         nml = names.value.decode().split("\n")
         idx, off = [], 0
         for i in range(rank):
@@ -163,7 +164,8 @@ def init_block(idx):
 class PBig:
     def __init__(self, L: PTensor, W1: PTensor, W2, R: PTensor):
         self._keep = (L, W1, W2, R)
-        self.h = _lib().tth_big_new(L.h, W1.h if W1 else None, W2.h if W2 else None, R.h)
+        self.h = _check(_lib().tth_big_new(L.h, W1.h if W1 else None, W2.h if W2 else None, R.h),
+                        "big_qtensor")
 
     def __del__(self):
         if getattr(self, "h", None) and T._lib is not None:
@@ -176,7 +178,7 @@ class PBig:
         return _ok(_lib().tth_big_expec(self.h, x.h), "expec")
 
     def size(self):
-        return _lib().tth_big_size(self.h)
+        return _ok(_lib().tth_big_size(self.h), "big_qtensor size")
 
     def heff(self, x: PTensor):
         """The device operator (ttg_heff) planned for x's indices."""
