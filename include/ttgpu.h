@@ -44,6 +44,11 @@ const char* ttg_last_error(void);
 int ttg_sync(void);
 /* bytes currently held by the device pool / high-water mark */
 int ttg_mem_info(size_t* in_use, size_t* peak, size_t* cached);
+/* fault injection for tests (no reference counterpart): the n-th device
+ * allocation request from now on is refused as if the pool were full
+ * (n >= 1; 0 disables).  Every entry point must then fail with an error
+ * before it launches work, and leave the library usable. */
+int ttg_pool_fail_after(int64_t n);
 
 /* ---- tensor store (qtensor storage, qtensor.h:84-92) ------------------- */
 /* allocate a tensor with the given block list, data zeroed */

@@ -114,7 +114,7 @@ def lib():
         L.orc_dmrg_bond_sample.argtypes = [_p, _p, _i, _d, _i, _d, _i, _i, _dp]
         L.orc_dmrg.restype = _d
         L.orc_dmrg.argtypes = [_p, _p, _i, _i, _ip, _i, _dp, _i, _ip, _d, _dp, C.POINTER(_d),
-                               C.POINTER(_d)]
+                               C.POINTER(_d), _dp, _dp, _ip]
         L.orc_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -350,9 +350,29 @@ def dmrg(psi: OQTT, H: OQTT, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12
     hs = np.zeros(2 * nsweeps)
     e0 = _d(0)
     sec = _d(0)
+    nb = 2 * nsweeps * (len(H) - 1)
+    bE, tr, kp = np.zeros(max(nb, 1)), np.zeros(max(nb, 1)), np.zeros(max(nb, 1), np.int32)
     E = lib().orc_dmrg(psi.h, H.h, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr, tol, hs,
-                       C.byref(e0), C.byref(sec))
+                       C.byref(e0), C.byref(sec), bE, tr, kp)
     return E, hs, e0.value, sec.value
+
+
+def dmrg_log(psi: OQTT, H: OQTT, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12):
+    """dmrg() with the per-bond log: a dict of the final energy, half-sweep
+    energies, initial energy, seconds and, per bond update in sweep order,
+    the Davidson energy, the truncation error and the kept bond dimension."""
+    mb = np.ascontiguousarray(np.array(maxbd, np.int32))
+    ct = np.ascontiguousarray(np.array(cutoff, np.float64))
+    mr = np.ascontiguousarray(np.array(max_restart, np.int32))
+    hs = np.zeros(2 * nsweeps)
+    e0 = _d(0)
+    sec = _d(0)
+    nb = 2 * nsweeps * (len(H) - 1)
+    bE, tr, kp = np.zeros(max(nb, 1)), np.zeros(max(nb, 1)), np.zeros(max(nb, 1), np.int32)
+    E = lib().orc_dmrg(psi.h, H.h, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr, tol, hs,
+                       C.byref(e0), C.byref(sec), bE, tr, kp)
+    return dict(E=E, halfsweep_E=hs, E0=e0.value, seconds=sec.value, bond_E=bE[:nb],
+                trunc_err=tr[:nb], kept=kp[:nb])
 
 
 def threads() -> int:

@@ -315,9 +315,10 @@ void orc_qtt_bond_dims(void* q, int* out) {
 }
 
 // DMRG (dmrg.cpp:760-813).  halfsweep_E has 2*nsweeps entries; returns E.
+// bond_E, trunc, kept (optional): per bond update, 2 (L-1) per sweep
 double orc_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int nc, const double* cut,
                 int nmr, const int* mr, double tol, double* halfsweep_E, double* init_E,
-                double* seconds) {
+                double* seconds, double* bond_E, double* trunc, int* kept) {
   try {
     std::vector<int> m(maxbd, maxbd + nmb), r(mr, mr + nmr);
     std::vector<double> c(cut, cut + nc);
@@ -327,6 +328,11 @@ double orc_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int 
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     for (size_t i = 0; i < log.halfsweep_energy.size(); i++) halfsweep_E[i] = log.halfsweep_energy[i];
+    for (size_t i = 0; i < log.bond_energy.size(); i++) {
+      if (bond_E) bond_E[i] = log.bond_energy[i];
+      if (trunc) trunc[i] = log.trunc_err[i];
+      if (kept) kept[i] = (int)log.kept[i];
+    }
     if (init_E) *init_E = log.initial_energy;
     return E;
   } catch (std::exception&) { g_err = "dmrg failed"; return 0; }

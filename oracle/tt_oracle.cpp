@@ -1417,7 +1417,11 @@ double dmrg(QTT& psi, QTT& H, int num_sweeps, const std::vector<int>& max_bd,
         SiteStats s = updateSite(psi, H, TR, TL, (unsigned)site, direction, max_bd[lbd],
                                  cutoff[lc], max_restart[lmr], tol, &rng);
         Energy = s.energy;
-        if (log) { log->bond_energy.push_back(s.energy); log->trunc_err.push_back(s.trunc_err); }
+        if (log) {
+          log->bond_energy.push_back(s.energy);
+          log->trunc_err.push_back(s.trunc_err);
+          log->kept.push_back(s.kept);
+        }
         updateEnv(psi, H, TR, TL, (unsigned)site, direction);
       }
       if (log) log->halfsweep_energy.push_back(Energy);

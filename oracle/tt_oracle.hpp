@@ -207,7 +207,7 @@ SiteStats updateSite(QTT& psi, QTT& H, std::vector<QTensor>& TR,
                      int max_bd, double cutoff, int max_restart, double tol,
                      uint64_t* rng);
 struct DmrgLog { std::vector<double> halfsweep_energy; std::vector<double> bond_energy;
-                 std::vector<double> trunc_err; double initial_energy = 0; };
+                 std::vector<double> trunc_err; std::vector<unsigned> kept; double initial_energy = 0; };
 double dmrg(QTT& psi, QTT& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart,
             double tol, DmrgLog* log);

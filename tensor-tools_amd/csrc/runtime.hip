@@ -28,6 +28,7 @@ struct Pool {
   std::map<void*, size_t> live;
   size_t in_use = 0, peak = 0, cached = 0;
   size_t budget = SIZE_MAX;
+  int64_t fail_at = 0;   // > 0: the fail_at-th request from now is refused (tests)
 } g_pool;
 
 // hand every cached buffer back to HIP (after the device is idle: a buffer
@@ -51,6 +52,10 @@ size_t size_class(size_t b) {
 }  // namespace
 
 void* dmalloc(size_t bytes) {
+  if (g_pool.fail_at > 0 && --g_pool.fail_at == 0) {
+    set_error("device allocation of " + std::to_string(bytes) + " bytes refused (injected failure)");
+    return nullptr;
+  }
   size_t sz = size_class(bytes);
   auto it = g_pool.free_list.lower_bound(sz);
   void* p = nullptr;
@@ -147,6 +152,12 @@ const char* ttg_last_error(void) { return g_error.c_str(); }
 int ttg_sync(void) {
   TTG_TRY(ensure_init());
   TTG_HIP(hipStreamSynchronize(g_rt.stream));
+  return 0;
+}
+
+int ttg_pool_fail_after(int64_t n) {
+  TTG_REQUIRE(n >= 0, "pool_fail_after: n must be >= 0");
+  g_pool.fail_at = n;
   return 0;
 }
 

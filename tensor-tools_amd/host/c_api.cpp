@@ -297,10 +297,12 @@ double tth_psiphi(void* psi, void* phi) {
 
 void tth_set_dmrg_progress(int every) { set_dmrg_progress(every); }
 
-// dmrg.cpp:760-813.  stats: [t_davidson, t_svd, t_env, n_matvec, seconds]
+// dmrg.cpp:702-755.  stats: [t_davidson, t_svd, t_env, n_matvec, seconds,
+// largest truncation error, sum over the last half-sweep]; bond_E, trunc,
+// kept (optional): per bond update in sweep order, 2 (L-1) per sweep
 double tth_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int nc, const double* cut,
                 int nmr, const int* mr, double tol, double* halfsweep_E, double* halfsweep_s,
-                double* init_E, double* stats) {
+                double* init_E, double* stats, double* bond_E, double* trunc, int* kept) {
   TTH_GUARD(kNaN,
     std::vector<int> m(maxbd, maxbd + nmb), r(mr, mr + nmr);
     std::vector<double> c(cut, cut + nc);
@@ -313,6 +315,11 @@ double tth_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int 
       if (halfsweep_s) halfsweep_s[i] = log.halfsweep_seconds[i];
     }
     if (init_E) *init_E = log.initial_energy;
+    for (size_t i = 0; i < log.bond_energy.size(); i++) {
+      if (bond_E) bond_E[i] = log.bond_energy[i];
+      if (trunc) trunc[i] = log.trunc_err[i];
+      if (kept) kept[i] = (int)log.kept[i];
+    }
     if (stats) {
       stats[0] = log.t_dav; stats[1] = log.t_svd; stats[2] = log.t_env;
       stats[3] = (double)log.n_matvec;

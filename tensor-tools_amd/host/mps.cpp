@@ -741,6 +741,7 @@ double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vecto
         if (log) {
           log->bond_energy.push_back(sl.energy);
           log->trunc_err.push_back(sl.trunc_err);
+          log->kept.push_back(sl.kept);
           log->t_dav += sl.t_dav;
           log->t_svd += sl.t_svd;
           log->t_env += now_s() - te;

@@ -273,6 +273,7 @@ void updateSite(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>
 struct DmrgLog {
   double initial_energy = 0;
   std::vector<double> halfsweep_energy, halfsweep_seconds, bond_energy, trunc_err;
+  std::vector<unsigned> kept;
   double t_dav = 0, t_svd = 0, t_env = 0;
   long n_matvec = 0;
 };

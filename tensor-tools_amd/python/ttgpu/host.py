@@ -53,7 +53,8 @@ def _lib():
             "tth_mps_free": ([_p], None), "tth_mpo_free": ([_p], None),
             "tth_mps_position": ([_p, _i], _d), "tth_mps_normalize": ([_p], _i),
             "tth_qtt_bond_dims": ([_p, _ip], None),
-            "tth_dmrg": ([_p, _p, _i, _i, _ip, _i, _dp, _i, _ip, _d, _dp, _dp, C.POINTER(_d), _dp], _d),
+            "tth_dmrg": ([_p, _p, _i, _i, _ip, _i, _dp, _i, _ip, _d, _dp, _dp, C.POINTER(_d), _dp,
+                          _dp, _dp, _ip], _d),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -312,8 +313,11 @@ def dmrg(psi: PMPS, H: PMPO, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12
     hsec = np.zeros(2 * nsweeps)
     e0 = _d(0)
     stats = np.zeros(7)
+    nb = 2 * nsweeps * (len(H) - 1)
+    bE, tr, kp = np.zeros(max(nb, 1)), np.zeros(max(nb, 1)), np.zeros(max(nb, 1), np.int32)
     E = _ok(_lib().tth_dmrg(psi.h, H.h, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr, tol, hs,
-                            hsec, C.byref(e0), stats), "dmrg")
+                            hsec, C.byref(e0), stats, bE, tr, kp), "dmrg")
     return dict(E=E, halfsweep_E=hs, halfsweep_s=hsec, E0=e0.value, t_davidson=stats[0],
                 t_svd=stats[1], t_env=stats[2], n_matvec=int(stats[3]), seconds=stats[4],
-                trunc_err_max=stats[5], trunc_err_last_halfsweep=stats[6])
+                trunc_err_max=stats[5], trunc_err_last_halfsweep=stats[6], bond_E=bE[:nb],
+                trunc_err=tr[:nb], kept=kp[:nb])

@@ -29,3 +29,32 @@ def mpo_bond(kind: str, site: int):
         raise ValueError(kind)
     H = O.mpo_autompo(N, terms)
     return H.tensor(site).to_flat(), H.tensor(site + 1).to_flat()
+
+
+def two_site_case(seed, zero_sector=None):
+    """A two-site bond: site tensors A = (l, s1, mid Out), B = (mid In, s2, r)
+    as svd_bond's templates, and a random two-site tensor x = (l, s1, s2, r)
+    over all legal blocks.  zero_sector: a left flux (q0 of l + s1) whose
+    blocks are set to zero, giving exactly zero singular values."""
+    import numpy as np
+    from ttgpu.flat import INWARD, LINK, OUTWARD, SITE, make_index, random_tensor
+
+    def sec(qs, ds):
+        return [((q, 0), d) for q, d in zip(qs, ds)]
+
+    rng = np.random.default_rng(seed)
+    l = make_index(INWARD, "ID1Link3", LINK, 0, sec([-4, -2, 0, 2, 4], [2, 4, 6, 4, 2]))
+    s1 = make_index(INWARD, "Site3", SITE, 0, sec([-1, 1], [1, 1]))
+    mo = make_index(OUTWARD, "ID1Link4", LINK, 0, sec([-5, -3, -1, 1, 3, 5], [1, 3, 6, 6, 3, 1]))
+    s2 = make_index(INWARD, "Site4", SITE, 0, sec([-1, 1], [1, 1]))
+    r = make_index(OUTWARD, "ID1Link5", LINK, 0, sec([-4, -2, 0, 2, 4], [1, 3, 5, 3, 1]))
+    A = random_tensor([l, s1, mo], rng)
+    B = random_tensor([mo.dag(), s2, r], rng)
+    x = random_tensor([l, s1, s2, r], rng)
+    if zero_sector is not None:
+        off = x.block_offsets()
+        for b in range(x.nblocks):
+            q = l.qn[x.block_qi[b, 0]][0] + s1.qn[x.block_qi[b, 1]][0]
+            if q == zero_sector:
+                x.data[off[b]:off[b + 1]] = 0.0
+    return A, B, x
