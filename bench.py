@@ -59,7 +59,10 @@ def sweep_leg(P, T, args):
     Davidson max_restart 2, tol 1e-12, from the same seeded random MPS
     (Gaussian Sz sectors of width sigma, m per link where the Hilbert space
     allows) in every precision form.  Reduced-precision forms end each
-    Davidson call with an FP64 product (Rayleigh quotient and residual)."""
+    Davidson call with the FP64 residual-correction step (two FP64 products
+    and a 2x2 FP64 Rayleigh-Ritz problem), so after one sweep from a random
+    state they are one Krylov step ahead of the FP64 form; the converged
+    comparison is converged_precision()."""
     from ttgpu.workloads import WORKLOADS, lattice_terms, random_mps
     lat, Nx, Ny, yper, J2, site, m_w, smax, sigma = WORKLOADS[args.workload]
     N = Nx * Ny
@@ -105,11 +108,62 @@ def sweep_leg(P, T, args):
     return out
 
 
+def converged_precision(P, T, args):
+    """The precision forms at convergence on the sweep leg's system: FP64
+    sweeps from the seeded random MPS until the energy per site changes by
+    less than --converge-tol between sweeps (at most --converge-sweeps), then
+    every form runs two more sweeps from that same converged state; reported:
+    each form's final energy per site and its difference from the FP64 run's,
+    beside the per-sweep FP64 energies that show the convergence."""
+    from ttgpu.workloads import WORKLOADS, lattice_terms, random_mps
+    lat, Nx, Ny, yper, J2, site, m_w, smax, sigma = WORKLOADS[args.workload]
+    N = Nx * Ny
+    m = args.sweep_m
+    H = P.mpo_autompo(N, lattice_terms(P, lat, Nx, Ny, yper, J2))
+    P.set_precision(T.FP64)
+    psi = P.mps_product([i % 2 for i in range(N)])
+    for i, t in enumerate(random_mps(N, m, sigma, seed=11)):
+        psi.set_tensor(i, P.PTensor.from_flat(t))
+    psi.set_center(-1)
+    hist = []
+    t0 = time.time()
+    for s in range(args.converge_sweeps):
+        r = P.dmrg(psi, H, 1, [m], [args.sweep_cutoff], [2], tol=1e-12)
+        hist.append(r["E"] / N)
+        log(f"[bench] converge: FP64 sweep {s + 1}: E/N = {hist[-1]:.13f}")
+        if len(hist) > 1 and abs(hist[-1] - hist[-2]) < args.converge_tol:
+            break
+    t_conv = time.time() - t0
+    start = [psi.tensor(i).to_flat() for i in range(N)]
+    out = {}
+    for name, prec in (("fp64", T.FP64), ("fp32", T.FP32), ("bf16x3", T.BF16X3)):
+        P.set_precision(prec)
+        q = P.mps_product([i % 2 for i in range(N)])
+        for i, t in enumerate(start):
+            q.set_tensor(i, P.PTensor.from_flat(t))
+        q.set_center(0)
+        r = P.dmrg(q, H, 2, [m], [args.sweep_cutoff], [2], tol=1e-12)
+        out[name] = {"E_per_site": r["E"] / N, "halfsweep_E_per_site": [float(v) / N for v in r["halfsweep_E"]],
+                     "trunc_err_last_halfsweep": r["trunc_err_last_halfsweep"],
+                     "sweep_s": round(float(np.sum(r["halfsweep_s"])) / 2, 4)}
+        del q
+    P.set_precision(T.FP64)
+    for v in out.values():
+        v["dE_per_site_vs_fp64"] = abs(v["E_per_site"] - out["fp64"]["E_per_site"])
+    conv = abs(hist[-1] - hist[-2]) if len(hist) > 1 else None
+    log("[bench] converged precision (6x12, m=%d): " % m + ", ".join(
+        f"{k} E/N {v['E_per_site']:.13f} |dE|/site vs fp64 {v['dE_per_site_vs_fp64']:.2e}" for k, v in out.items()))
+    return {"system": f"{args.workload} lattice, m={m}, cutoff {args.sweep_cutoff}", "fp64_sweeps": len(hist),
+            "fp64_E_per_site_by_sweep": hist, "last_sweep_change_per_site": conv,
+            "converge_tol": args.converge_tol, "fp64_converge_s": round(t_conv, 2),
+            "forms_two_more_sweeps": out}
+
+
 def precision_check(P, T):
     """Converged comparison of the precision forms: the N=20 open Heisenberg
     chain, two-site DMRG from the Neel state, 10 sweeps, bond dimension
-    16 -> 64, cutoff 1e-10, Davidson max_restart 2, tol 1e-10; the final
-    energy per site of each form against the FP64 run and against exact
+    16, 32, 48, then 64 for the last seven sweeps, cutoff 1e-10, Davidson
+    max_restart 2, tol 1e-10; the final energy per site of each form against the FP64 run and against exact
     diagonalisation (tests/golden/ed_energies.json)."""
     N = 20
     ed = json.load(open(os.path.join(ROOT, "tests", "golden", "ed_energies.json")))["heisenberg_obc_N20"]
@@ -118,7 +172,7 @@ def precision_check(P, T):
     for name, prec in (("fp64", T.FP64), ("fp32", T.FP32), ("bf16x3", T.BF16X3)):
         P.set_precision(prec)
         psi = P.mps_product([i % 2 for i in range(N)])
-        r = P.dmrg(psi, H, 10, [16, 32, 64, 64], [1e-10], [2], 1e-10)
+        r = P.dmrg(psi, H, 10, [16, 32, 48, 64, 64, 64, 64, 64, 64, 64], [1e-10], [2], 1e-10)
         out[name] = {"E_per_site": r["E"] / N, "dE_per_site_vs_ed": abs(r["E"] - ed) / N}
     P.set_precision(T.FP64)
     for v in out.values():
@@ -126,7 +180,7 @@ def precision_check(P, T):
     log("[bench] precision check (N=20 chain, 10 sweeps): " + ", ".join(
         f"{k} |dE|/site vs fp64 {v['dE_per_site_vs_fp64']:.2e} vs ED {v['dE_per_site_vs_ed']:.2e}"
         for k, v in out.items()))
-    return {"system": "Heisenberg chain N=20 OBC, 10 sweeps, m 16->64, cutoff 1e-10", "forms": out}
+    return {"system": "Heisenberg chain N=20 OBC, 10 sweeps, m 16,32,48,64x7, cutoff 1e-10", "forms": out}
 
 
 def cpu_bond_sample(args, N):
@@ -177,6 +231,10 @@ def main():
     ap.add_argument("--sweep-forms", default="fp64,bf16x3,fp32")
     ap.add_argument("--sweeps", type=int, default=1, help="sweeps of the sweep leg (sweep_s is per sweep)")
     ap.add_argument("--sweep-cutoff", type=float, default=1e-9, help="SVD cutoff of the sweep leg")
+    ap.add_argument("--converge-sweeps", type=int, default=16,
+                    help="most FP64 sweeps of the converged precision comparison (0 skips it)")
+    ap.add_argument("--converge-tol", type=float, default=1e-10,
+                    help="energy-per-site change between sweeps that counts as converged")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a dispatch of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -280,6 +338,8 @@ def main():
     sweep = sweep_leg(P, T, args) if args.sweep_m > 0 else None
     if sweep is not None:
         sweep["precision_check"] = precision_check(P, T)
+        if args.converge_sweeps > 0:
+            sweep["precision_converged"] = converged_precision(P, T, args)
 
     lat, Nx, Ny, yper, J2, site, m, smax, sigma = WORKLOADS[args.workload]
     out = {

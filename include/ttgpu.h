@@ -135,7 +135,33 @@ int ttg_heff_stage_time(ttg_heff h, ttg_tensor x, int n, double* ms3);
  * 32x32, 64x32, 32x64, 128x64, 256x16, 128x16, 64x16, 96x96, 96x32, 32x96;
  * 11: the vector-ALU kernel for K <= 32, N <= 16) and its work-item count */
 int ttg_heff_tiles(ttg_heff h, int* tile_cfg3, int* ntiles3);
+/* the plan of stage 0, 1 or 2 (L*x, *W12, *R), borrowed: valid while h
+ * lives and never passed to ttg_plan_free.  For the bit-exact checks of each
+ * stage's block list, offsets and task list (ttg_plan_info,
+ * ttg_plan_tasks_download, ttg_plan_new_output), the three operator* calls of
+ * big_qtensor::product (big_qtensor.cpp:27-39). */
+int ttg_heff_stage_plan(ttg_heff h, int stage, ttg_plan* out);
 int ttg_heff_free(ttg_heff h);
+
+/* ---- environment update (updateEnv, dmrg.cpp:491-509; buildEnv's steps,
+ * dmrg.cpp:76-83) ------------------------------------------------------
+ * env' = ((psi^dag' * env) * W) * psi in three planned grouped-GEMM stages,
+ * for the reference's index layouts:
+ *   psi = (l In, s In, r Out); W = (w_l In, s Out lvl0, s In lvl1, w_r Out);
+ *   MoveFromLeft:  env = TL[i]   = (l' In lvl1, w_l Out, l Out) -> TL[i+1]
+ *   MoveFromRight: env = TR[i]   = (r' Out lvl1, w_r In, r In)  -> TR[i-1]
+ * psi^dag' is psi with every arrow flipped (and primed, a tag change only);
+ * its data is psi's.  Operands whose paired indices do not carry equal
+ * sectors with opposite arrows are rejected before any launch.  prec selects
+ * the arithmetic of the three stages (FP64 output in every form).  *out is a
+ * new tensor with the block list operator* gives the last stage. */
+int ttg_env_update(ttg_tensor psi, ttg_tensor env, ttg_tensor W, int direction, int prec,
+                   ttg_tensor* out);
+/* the (psi^dag'*env, *W, *psi) index pairs ttg_env_update uses for a
+ * direction, in ttg_heff_create's pairs3 encoding (host only, no device
+ * work): the pairs qtensor::operator* finds by tag (qtensor.cpp:540-555) for
+ * the layouts above */
+int ttg_env_pairs(int direction, int* pairs3);
 
 /* ---- sector SVD with global truncation (qtensor_op.cpp:1097-1453) -------
  * A's first nleft indices (in order) form the rows, the rest the columns

@@ -120,8 +120,8 @@ struct TuneKey {
 static std::map<TuneKey, int> g_tuned;
 constexpr double kTuneMinFlops = 2e8;
 
-static int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, bool out_f64,
-                      hipStream_t s) {
+int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, bool out_f64,
+               hipStream_t s) {
   if (p->cfg == TILE_SKINNY || p->ntasks == 0 || p->flops < kTuneMinFlops) return 0;
   const char* e = getenv("TTG_TILE_CFG");
   if (e && *e) return 0;
@@ -356,6 +356,12 @@ int ttg_heff_tiles(ttg_heff h, int* tile_cfg3, int* ntiles3) {
     if (tile_cfg3) tile_cfg3[k] = h->p[k]->cfg;
     if (ntiles3) ntiles3[k] = h->p[k]->ntiles;
   }
+  return 0;
+}
+
+int ttg_heff_stage_plan(ttg_heff h, int stage, ttg_plan* out) {
+  TTG_REQUIRE(h && out && stage >= 0 && stage < 3, "heff_stage_plan: bad argument");
+  *out = h->p[stage];
   return 0;
 }
 

@@ -247,6 +247,29 @@ void fill_f64(double* d, size_t n, double v) {
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, stream(), d, n, v);
 }
 
+// the same sectors and block list with every arrow flipped (qtensor::dag,
+// qtensor.cpp:1575-1581): a new structure object, the blocks copied on the
+// device
+int meta_dag(const TMeta& o, MetaPtr* out) {
+  auto m = std::make_shared<TMeta>();
+  m->rank = o.rank;
+  m->h_idx = o.h_idx;
+  for (int j = 0; j <= MAXR; j++) m->idx_off[j] = o.idx_off[j];
+  for (int j = 0; j < o.rank; j++) {
+    m->arrow[j] = 1 - o.arrow[j];
+    m->nsec[j] = o.nsec[j];
+  }
+  TTG_TRY(upload_idx(*m));
+  TTG_TRY(meta_alloc_blocks(*m, o.nblocks, o.total));
+  if (o.nblocks)
+    TTG_HIP(hipMemcpyAsync(m->d_qi, o.d_qi, sizeof(int) * (size_t)o.nblocks * o.rank,
+                           hipMemcpyDeviceToDevice, stream()));
+  TTG_HIP(hipMemcpyAsync(m->d_off, o.d_off, sizeof(int64_t) * ((size_t)o.nblocks + 1),
+                         hipMemcpyDeviceToDevice, stream()));
+  *out = m;
+  return 0;
+}
+
 int tensor_f32(ttg_tensor t, const float** out) {
   if (!t->d_f32) {
     t->d_f32 = dalloc<float>(t->m->total);
@@ -417,23 +440,8 @@ int ttg_tensor_index_download(ttg_tensor t, int j, int* nsec, int* qn, int* qdim
 
 int ttg_tensor_dag(ttg_tensor t) {
   TTG_REQUIRE(t, "null tensor");
-  const TMeta& o = *t->m;
-  // new structure object: same sectors and blocks, flipped arrows
-  auto m = std::make_shared<TMeta>();
-  m->rank = o.rank;
-  m->h_idx = o.h_idx;
-  for (int j = 0; j <= MAXR; j++) m->idx_off[j] = o.idx_off[j];
-  for (int j = 0; j < o.rank; j++) {
-    m->arrow[j] = 1 - o.arrow[j];
-    m->nsec[j] = o.nsec[j];
-  }
-  TTG_TRY(upload_idx(*m));
-  TTG_TRY(meta_alloc_blocks(*m, o.nblocks, o.total));
-  if (o.nblocks)
-    TTG_HIP(hipMemcpyAsync(m->d_qi, o.d_qi, sizeof(int) * (size_t)o.nblocks * o.rank,
-                           hipMemcpyDeviceToDevice, stream()));
-  TTG_HIP(hipMemcpyAsync(m->d_off, o.d_off, sizeof(int64_t) * ((size_t)o.nblocks + 1),
-                         hipMemcpyDeviceToDevice, stream()));
+  MetaPtr m;
+  TTG_TRY(meta_dag(*t->m, &m));
   t->m = m;
   return 0;
 }

@@ -148,6 +148,8 @@ namespace ttg {
 int tensor_from_meta(const MetaPtr& m, bool zero, ttg_tensor* out);
 inline void mark_written(ttg_tensor t) { t->version++; }
 int tensor_f32(ttg_tensor t, const float** out);
+// flipped-arrow copy of a block structure (qtensor::dag)
+int meta_dag(const TMeta& o, MetaPtr* out);
 // same index sectors and block list (hence the same packed offsets); equal
 // pointers short-cut, otherwise the block lists are compared on the host
 bool same_meta(const MetaPtr& a, const MetaPtr& b);

@@ -118,4 +118,9 @@ int gemm_occupancy(int prec, int cfg, bool a_mcontig, bool b_kcontig);
 int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb);
 // the same plan's work list rebuilt for another MFMA tile shape
 int rebuild_tiles(ttg_plan_s& p, int cfg);
+// measured tile-shape choice of one stage of a multi-stage operator (the
+// operands are the stage's own; C is overwritten); `stage` keys the cache of
+// choices together with the stage's precision, size and operand layout
+int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, bool out_f64,
+               hipStream_t s);
 }  // namespace ttg

@@ -59,6 +59,7 @@ const char* tth_last_error() { return g_err.c_str(); }
 
 int tth_init(int device) { return ttg_init(device); }
 void tth_set_precision(int prec) { set_heff_precision(prec); }
+void tth_set_env_precision(int prec) { set_env_precision(prec); }
 
 void* tth_tensor_import(int rank, const int* arrows, const int* levels, const int* types,
                         const char* names, const int* nsec, const int* qn, const int* qdim,
@@ -278,6 +279,9 @@ int tth_qtt_length(void* q) { return (int)((qMPS<double>*)q)->length; }
 int tth_qtt_center(void* q) { return ((qMPS<double>*)q)->center; }
 // -1 marks the tensor train as not canonical (the next position() runs rc())
 void tth_qtt_set_center(void* q, int c) { ((qMPS<double>*)q)->center = c; }
+// empty trains, filled by load (checkpoint.cpp)
+void* tth_mps_new() { TTH_GUARD(nullptr, return new qMPS<double>();) }
+void* tth_mpo_new() { TTH_GUARD(nullptr, return new qMPO<double>();) }
 void tth_mps_free(void* q) { delete (qMPS<double>*)q; }
 void tth_mpo_free(void* q) { delete (qMPO<double>*)q; }
 double tth_mps_position(void* q, int site) { TTH_GUARD(kNaN, return ((qMPS<double>*)q)->position(site);) }
@@ -297,43 +301,95 @@ double tth_psiphi(void* psi, void* phi) {
 
 void tth_set_dmrg_progress(int every) { set_dmrg_progress(every); }
 
-// dmrg.cpp:702-755.  stats: [t_davidson, t_svd, t_env, n_matvec, seconds,
-// largest truncation error, sum over the last half-sweep]; bond_E, trunc,
-// kept (optional): per bond update in sweep order, 2 (L-1) per sweep
+// stats: [t_davidson, t_svd, t_env, n_matvec, seconds, largest truncation
+// error, sum over the last half-sweep, largest device-pool live bytes after a
+// bond]; bond_E, trunc, kept, env (optional): per bond update in sweep order,
+// `per` bonds per half-sweep (env: elements of the environment it produced)
+static void copy_log(const DmrgLog& log, double secs, size_t per, double* halfsweep_E,
+                     double* halfsweep_s, double* init_E, double* stats, double* bond_E, double* trunc,
+                     int* kept, double* env) {
+  for (size_t i = 0; i < log.halfsweep_energy.size(); i++) {
+    halfsweep_E[i] = log.halfsweep_energy[i];
+    if (halfsweep_s) halfsweep_s[i] = log.halfsweep_seconds[i];
+  }
+  if (init_E) *init_E = log.initial_energy;
+  for (size_t i = 0; i < log.bond_energy.size(); i++) {
+    if (bond_E) bond_E[i] = log.bond_energy[i];
+    if (trunc) trunc[i] = log.trunc_err[i];
+    if (kept) kept[i] = (int)log.kept[i];
+    if (env) env[i] = (double)log.env_elems[i];
+  }
+  if (stats) {
+    stats[0] = log.t_dav; stats[1] = log.t_svd; stats[2] = log.t_env;
+    stats[3] = (double)log.n_matvec;
+    stats[4] = secs;
+    double tmax = 0, tlast = 0;
+    for (double t : log.trunc_err) tmax = std::max(tmax, t);
+    const size_t nb = log.trunc_err.size();
+    for (size_t i = (nb > per ? nb - per : 0); i < nb; i++) tlast += log.trunc_err[i];
+    stats[5] = tmax;
+    stats[6] = tlast;
+    stats[7] = (double)log.pool_live_max;
+  }
+}
+
+static double secs_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// dmrg.cpp:702-755 (schedules cycled per sweep)
 double tth_dmrg(void* psi, void* H, int nsweeps, int nmb, const int* maxbd, int nc, const double* cut,
                 int nmr, const int* mr, double tol, double* halfsweep_E, double* halfsweep_s,
-                double* init_E, double* stats, double* bond_E, double* trunc, int* kept) {
+                double* init_E, double* stats, double* bond_E, double* trunc, int* kept, double* env) {
   TTH_GUARD(kNaN,
     std::vector<int> m(maxbd, maxbd + nmb), r(mr, mr + nmr);
     std::vector<double> c(cut, cut + nc);
     DmrgLog log;
     auto t0 = std::chrono::steady_clock::now();
-    double E = dmrg(*(qMPS<double>*)psi, *(qMPO<double>*)H, nsweeps, m, c, r, tol, &log);
-    auto t1 = std::chrono::steady_clock::now();
-    for (size_t i = 0; i < log.halfsweep_energy.size(); i++) {
-      halfsweep_E[i] = log.halfsweep_energy[i];
-      if (halfsweep_s) halfsweep_s[i] = log.halfsweep_seconds[i];
-    }
-    if (init_E) *init_E = log.initial_energy;
-    for (size_t i = 0; i < log.bond_energy.size(); i++) {
-      if (bond_E) bond_E[i] = log.bond_energy[i];
-      if (trunc) trunc[i] = log.trunc_err[i];
-      if (kept) kept[i] = (int)log.kept[i];
-    }
-    if (stats) {
-      stats[0] = log.t_dav; stats[1] = log.t_svd; stats[2] = log.t_env;
-      stats[3] = (double)log.n_matvec;
-      stats[4] = std::chrono::duration<double>(t1 - t0).count();
-      // largest truncation error of any bond, and the sum over the last half-sweep's bonds
-      double tmax = 0, tlast = 0;
-      for (double t : log.trunc_err) tmax = std::max(tmax, t);
-      const size_t nb = log.trunc_err.size(), per = psi ? ((qMPS<double>*)psi)->length - 1 : 0;
-      for (size_t i = (nb > per ? nb - per : 0); i < nb; i++) tlast += log.trunc_err[i];
-      stats[5] = tmax;
-      stats[6] = tlast;
-    }
+    auto* q = (qMPS<double>*)psi;
+    double E = dmrg(*q, *(qMPO<double>*)H, nsweeps, m, c, r, tol, &log);
+    copy_log(log, secs_since(t0), q->length - 1, halfsweep_E, halfsweep_s, init_E, stats, bond_E, trunc,
+             kept, env);
     return E;
   )
 }
+
+// dmrg.cpp:760-813 (bonds of sites [start, stop])
+double tth_dmrg_range(void* psi, void* H, int start, int stop, int nsweeps, int nmb, const int* maxbd,
+                      int nc, const double* cut, int nmr, const int* mr, double tol, double* halfsweep_E,
+                      double* halfsweep_s, double* init_E, double* stats, double* bond_E, double* trunc,
+                      int* kept, double* env) {
+  TTH_GUARD(kNaN,
+    if (start < 0 || stop <= start) fail("dmrg: bad site range");
+    std::vector<int> m(maxbd, maxbd + nmb), r(mr, mr + nmr);
+    std::vector<double> c(cut, cut + nc);
+    DmrgLog log;
+    auto t0 = std::chrono::steady_clock::now();
+    double E = dmrg(*(qMPS<double>*)psi, *(qMPO<double>*)H, (unsigned)start, (unsigned)stop, nsweeps, m, c,
+                    r, tol, &log);
+    copy_log(log, secs_since(t0), (size_t)(stop - start), halfsweep_E, halfsweep_s, init_E, stats, bond_E,
+             trunc, kept, env);
+    return E;
+  )
+}
+
+// dmrg.cpp:653-697 (half-sweeps start_sweep .. num_half-1; the resume form)
+double tth_dmrg_resume(void* psi, void* H, int num_half, int max_bd, double cutoff, int max_restart,
+                       int start_sweep, double tol, double* halfsweep_E, double* halfsweep_s,
+                       double* init_E, double* stats, double* bond_E, double* trunc, int* kept,
+                       double* env) {
+  TTH_GUARD(kNaN,
+    DmrgLog log;
+    auto t0 = std::chrono::steady_clock::now();
+    auto* q = (qMPS<double>*)psi;
+    double E = dmrg(*q, *(qMPO<double>*)H, num_half, max_bd, cutoff, 'S', 3, max_restart, start_sweep,
+                    tol, &log);
+    copy_log(log, secs_since(t0), q->length - 1, halfsweep_E, halfsweep_s, init_E, stats, bond_E, trunc,
+             kept, env);
+    return E;
+  )
+}
+
+void tth_set_env_checkpoint_stride(int k) { set_env_checkpoint_stride(k < 1 ? 1 : k); }
 
 }  // extern "C"

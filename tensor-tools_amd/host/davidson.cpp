@@ -119,19 +119,52 @@ double tensor_davidsonIT(big_qtensor<double>& A, qtensor<double>& x_in, int m, i
     }
   }
   x.normalize();
-  // Reduced-precision forms: one FP64 product gives the Rayleigh quotient
-  // and residual of the final vector in FP64.  The quotient's error is
-  // second order in the vector's error, so the FP32-level accuracy of the
-  // reduced-precision iteration becomes an energy accurate to ~1e-12 relative.
+  // Reduced-precision forms: an FP64 residual-correction step.  The FP64
+  // product y = H x of the reduced-precision Ritz vector gives the FP64
+  // residual r = y - <x,y> x; r (orthonormalised against x) extends the
+  // basis to {x, r}, whose 2x2 Rayleigh-Ritz problem is solved in FP64 with
+  // a second FP64 product H r.  The corrected vector c0 x + c1 r goes on to
+  // the SVD, and its FP64 Rayleigh quotient is the energy: the step removes
+  // the component of the reduced-precision error along the residual, and the
+  // quotient's error is second order in what remains.
   const double lambda_reduced = lambda;
   bool corrected = false;
   if (heff_precision() != TTG_FP64) {
     qtensor<double> y;
     A.product_into_fp64(x, y);
     nmv++;
-    lambda = y.inner_product(x);
-    y.add(x, -lambda);
-    qnorm = y.norm();
+    const double a = y.inner_product(x);
+    qtensor<double> r = y;
+    r.add(x, -a);
+    r.add(x, -r.inner_product(x));   // re-orthogonalise against x (rounding)
+    const double rn = r.norm();
+    lambda = a;
+    qnorm = rn;
+    if (rn > 1e-14 * std::max(1.0, std::fabs(a))) {
+      r *= 1.0 / rn;
+      qtensor<double> z;
+      A.product_into_fp64(r, z);
+      nmv++;
+      const double b = z.inner_product(x), d = z.inner_product(r);
+      // lowest eigenpair of [[a, b], [b, d]]: lambda = (a+d)/2 - sqrt(((a-d)/2)^2 + b^2),
+      // eigenvector (d - lambda, -b) with a non-negative x component
+      const double h = 0.5 * (a - d);
+      const double lam = 0.5 * (a + d) - std::sqrt(h * h + b * b);
+      double c0 = d - lam, c1 = -b;
+      const double cn = std::sqrt(c0 * c0 + c1 * c1);
+      if (cn > 0) {
+        c0 /= cn;
+        c1 /= cn;
+        x *= c0;
+        x.add(r, c1);
+        y *= c0;
+        y.add(z, c1);
+        x.normalize();
+        lambda = lam;
+        y.add(x, -lambda);
+        qnorm = y.norm();
+      }
+    }
     corrected = true;
   }
   x_in = std::move(x);

@@ -438,6 +438,14 @@ double calcEntropy(qtensor<double>& S, double cutoff) {
 // progress lines on stderr every g_progress_every sites or bonds (0: silent)
 static int g_progress_every = 0;
 void set_dmrg_progress(int every) { g_progress_every = every; }
+// environments kept between half-sweeps: every g_env_stride-th site (and the
+// range ends); the others are recomputed from the nearest kept one when a
+// bond needs them (1: keep every environment, one full set)
+static int g_env_stride = 1;
+void set_env_checkpoint_stride(int k) {
+  if (k < 1) fail("env checkpoint stride must be >= 1");
+  g_env_stride = k;
+}
 
 // ---------------------------------------------------------------- canonical form
 static void rename_like(qtensor_index& ix, const qtensor_index& mid) {
@@ -540,6 +548,39 @@ template class qTensorTrain<double, 1>;
 template class qTensorTrain<double, 2>;
 
 // ---------------------------------------------------------------- DMRG
+// One environment step ((psi^dag' * env) * W) * psi (dmrg.cpp:494-508, 76-83).
+// The index pairs of the three operator* calls are found by tag, as the
+// reference does; when they are the reference layouts' pairs (every DMRG
+// environment), the step runs as one ttg_env_update call, else as three
+// contractions.
+static qtensor<double> env_step(const qtensor<double>& psi, const qtensor<double>& env,
+                                const qtensor<double>& W, int dir) {
+  qtensor<double> t1 = psi;
+  t1.conj(); t1.dag(); t1.prime();
+  IndexPairing p1 = pair_indices(t1.idx_set, env.idx_set);
+  IndexPairing p2 = pair_indices(p1.result, W.idx_set);
+  IndexPairing p3 = pair_indices(p2.result, psi.idx_set);
+  int want[16];
+  check(ttg_env_pairs(dir, want), "env pairs");
+  bool std_layout = psi._initted && env._initted && W._initted;
+  const IndexPairing* ps[3] = {&p1, &p2, &p3};
+  int k = 0;
+  for (int s = 0; s < 3 && std_layout; s++) {
+    const int n = want[k++];
+    std_layout = (int)ps[s]->pa.size() == n;
+    for (int i = 0; i < n && std_layout; i++, k += 2)
+      std_layout = ps[s]->pa[i] == want[k] && ps[s]->pb[i] == want[k + 1];
+  }
+  if (!std_layout) return ((t1 * env) * W) * psi;
+  qtensor<double> res(p3.result);
+  ttg_tensor out = nullptr;
+  check(ttg_env_update(psi.dev.get(), env.dev.get(), W.dev.get(), dir, env_precision(), &out),
+        "env_update");
+  res.dev = adopt(out);
+  res._initted = true;
+  return res;
+}
+
 // dmrg.cpp:56-84
 void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
               std::vector<qtensor<double>>& TL) {
@@ -557,28 +598,68 @@ void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& 
     TR[L - 1] = std::move(re);
   }
   for (unsigned i = L - 1; i > 0; i--) {
-    qtensor<double> t1 = psi.A[i];
-    t1.conj(); t1.dag(); t1.prime();
-    qtensor<double> t4 = t1 * TR[i];
-    qtensor<double> t5 = t4 * H.A[i];
-    TR[i - 1] = t5 * psi.A[i];
+    TR[i - 1] = env_step(psi.A[i], TR[i], H.A[i], MoveFromRight);
+    // with a checkpoint stride, only every k-th environment and the end stay
+    if (g_env_stride > 1 && i < L - 1 && i % (unsigned)g_env_stride != 0) TR[i] = qtensor<double>();
+  }
+}
+
+// The mirror of buildEnv: the open-boundary ends TL[0] and TR[L-1], then
+// TL[i+1] for every site from the left (the environments a sweep that
+// starts at the right end reads).
+static void buildEnvLeft(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+                         std::vector<qtensor<double>>& TL) {
+  const unsigned L = psi.length;
+  {
+    qtensor<double> le({psi.A[0].idx_set[0], H.A[0].idx_set[0], psi.A[0].idx_set[0]});
+    le.idx_set[0].primeLink(); le.idx_set[0].dag();
+    le.initBlock(); le.setOne(); le.dag();
+    TL[0] = std::move(le);
+  }
+  {
+    qtensor<double> re({psi.A[L - 1].idx_set.back(), H.A[L - 1].idx_set.back(), psi.A[L - 1].idx_set.back()});
+    re.idx_set[0].primeLink(); re.idx_set[0].dag();
+    re.initBlock(); re.setOne(); re.dag();
+    TR[L - 1] = std::move(re);
+  }
+  for (unsigned i = 0; i + 1 < L; i++) {
+    TL[i + 1] = env_step(psi.A[i], TL[i], H.A[i], MoveFromLeft);
+    if (g_env_stride > 1 && i > 0 && i % (unsigned)g_env_stride != 0) TL[i] = qtensor<double>();
+  }
+}
+
+// dmrg.cpp:89-134: TL[start] holds sites 0..start-1 and TR[stop] sites
+// stop+1..L-1, each contracted into one end tensor; TR[i-1] for
+// stop >= i > start as in the full build
+void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+              std::vector<qtensor<double>>& TL, unsigned start, unsigned stop) {
+  const unsigned L = psi.length;
+  if (!(start < stop && stop < L) || TR.size() < L || TL.size() < L) fail("buildEnv: bad site range");
+  {
+    qtensor<double> le({psi.A[0].idx_set[0], H.A[0].idx_set[0], psi.A[0].idx_set[0]});
+    le.idx_set[0].primeLink(); le.idx_set[0].dag();
+    le.initBlock(); le.setOne(); le.dag();
+    TL[start] = std::move(le);
+  }
+  for (unsigned i = 0; i < start; i++) TL[start] = env_step(psi.A[i], TL[start], H.A[i], MoveFromLeft);
+  {
+    qtensor<double> re({psi.A[L - 1].idx_set.back(), H.A[L - 1].idx_set.back(), psi.A[L - 1].idx_set.back()});
+    re.idx_set[0].primeLink(); re.idx_set[0].dag();
+    re.initBlock(); re.setOne(); re.dag();
+    TR[stop] = std::move(re);
+  }
+  for (unsigned i = L - 1; i > stop; i--) TR[stop] = env_step(psi.A[i], TR[stop], H.A[i], MoveFromRight);
+  for (unsigned i = stop; i > start; i--) {
+    TR[i - 1] = env_step(psi.A[i], TR[i], H.A[i], MoveFromRight);
+    if (g_env_stride > 1 && i < stop && i % (unsigned)g_env_stride != 0) TR[i] = qtensor<double>();
   }
 }
 
 // dmrg.cpp:491-509
 void updateEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
                std::vector<qtensor<double>>& TL, const unsigned& site, int& direction) {
-  qtensor<double> t1 = psi.A[site];
-  t1.conj(); t1.dag(); t1.prime();
-  if (direction == MoveFromLeft) {
-    qtensor<double> t4 = t1 * TL[site];
-    qtensor<double> t5 = t4 * H.A[site];
-    TL[site + 1] = t5 * psi.A[site];
-  } else {
-    qtensor<double> t4 = t1 * TR[site];
-    qtensor<double> t5 = t4 * H.A[site];
-    TR[site - 1] = t5 * psi.A[site];
-  }
+  if (direction == MoveFromLeft) TL[site + 1] = env_step(psi.A[site], TL[site], H.A[site], MoveFromLeft);
+  else TR[site - 1] = env_step(psi.A[site], TR[site], H.A[site], MoveFromRight);
 }
 
 static double now_s() {
@@ -681,82 +762,175 @@ double psiphi(qMPS<double>& psi, qMPS<double>& phi) {
   return res;
 }
 
+// Initial energy <psi|H|psi> at the centre site c (dmrg.cpp:717-724).
+static double initial_energy(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+                             std::vector<qtensor<double>>& TL, unsigned c, DmrgLog* log) {
+  big_qtensor<double> A;
+  A.setLeft(&TL[c]);
+  A.addMid(&H.A[c]);
+  A.setRight(&TR[c]);
+  const double E = A.expec(psi.A[c]);
+  if (log) log->initial_energy = E;
+  if (g_progress_every > 0) {
+    fprintf(stderr, "[dmrg] canonical form and environments ready, E0=%.12f\n", E);
+    fflush(stderr);
+  }
+  return E;
+}
+
+static size_t env_elems(const qtensor<double>& t) { return t.dev ? t.total_size() : 0; }
+
+// Make sure the environments bond (a, a+1) reads exist.  With a checkpoint
+// stride above 1 the environments between kept sites were released after
+// their last use of the previous half-sweep; they are recomputed here from
+// the nearest kept one on the far side of the bond, from site tensors that
+// have not changed since (bit-identical to the released ones).  The
+// intermediate ones stay: the next bonds of the half-sweep read them.
+static void ensure_env(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+                       std::vector<qtensor<double>>& TL, unsigned a, unsigned lo, unsigned hi,
+                       int direction) {
+  if (direction == MoveFromLeft) {
+    if (TR[a + 1].dev) return;
+    unsigned j = a + 2;
+    while (j <= hi && !TR[j].dev) j++;
+    if (j > hi) fail("dmrg: no kept right environment to rebuild from");
+    for (unsigned t = j; t > a + 1; t--) TR[t - 1] = env_step(psi.A[t], TR[t], H.A[t], MoveFromRight);
+  } else {
+    if (TL[a].dev) return;
+    if (a == 0) fail("dmrg: left end environment missing");
+    int j = (int)a - 1;
+    while (j >= (int)lo && !TL[j].dev) j--;
+    if (j < (int)lo) fail("dmrg: no kept left environment to rebuild from");
+    for (unsigned t = (unsigned)j; t < a; t++) TL[t + 1] = env_step(psi.A[t], TL[t], H.A[t], MoveFromLeft);
+  }
+}
+
+// One half-sweep over the bonds of [lo, hi] (dmrg.cpp:731-740, 785-792):
+// MoveFromLeft visits sites lo .. hi-1, MoveFromRight sites hi .. lo+1.
+static void half_sweep(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+                       std::vector<qtensor<double>>& TL, int direction, unsigned lo, unsigned hi,
+                       int max_bd, double cutoff, int max_restart, double tol, double& Energy,
+                       int sweep_label, DmrgLog* log) {
+  const int search_space_size = 3;
+  const double t0 = now_s();
+  const unsigned nb = hi - lo;
+  const unsigned k = (unsigned)g_env_stride;
+  for (unsigned i = 0; i < nb; i++) {
+    const unsigned site = (direction == MoveFromLeft) ? lo + i : hi - i;
+    const unsigned a = (direction == MoveFromLeft) ? site : site - 1;
+    const double tr0 = now_s();
+    ensure_env(psi, H, TR, TL, a, lo, hi, direction);
+    const double t_rebuild = now_s() - tr0;
+    SiteLog sl{};
+    updateSite(psi, H, TR, TL, site, Energy, direction, max_bd, cutoff, 'S', search_space_size,
+               max_restart, tol, &sl);
+    const double te = now_s();
+    updateEnv(psi, H, TR, TL, site, direction);
+    // The environment this bond read on the far side is stale: the next
+    // half-sweep recomputes it before any bond reads it.  Releasing it
+    // keeps one environment set on the device instead of two (results are
+    // unchanged; the range's ends TL[lo] and TR[hi] are kept).  With a
+    // checkpoint stride k > 1, the environment this bond read on the near
+    // side (TL[site] / TR[site]) is kept for the next half-sweep only at
+    // sites that are multiples of k.
+    if (direction == MoveFromLeft) {
+      if (site + 1 < hi) TR[site + 1] = qtensor<double>();
+      if (site == lo) TR[lo] = qtensor<double>();
+      if (k > 1 && site > lo && site % k != 0) TL[site] = qtensor<double>();
+    } else {
+      if (site - 1 > lo) TL[site - 1] = qtensor<double>();
+      if (site == hi) TL[hi] = qtensor<double>();
+      if (k > 1 && site < hi && site % k != 0) TR[site] = qtensor<double>();
+    }
+    ttg_sync();
+    if (g_progress_every > 0 && (i % g_progress_every == 0 || i + 1 == nb)) {
+      size_t in_use = 0, peak = 0, cached = 0;
+      ttg_mem_info(&in_use, &peak, &cached);
+      fprintf(stderr, "[dmrg] sweep %d half %d bond %u/%u E=%.12f kept=%u %.2fs (device pool %.1f GB live, "
+              "%.1f GB cached)\n", sweep_label, direction == MoveFromLeft ? 0 : 1, i + 1, nb, sl.energy,
+              sl.kept, now_s() - t0, in_use / 1e9, cached / 1e9);
+      fflush(stderr);
+    }
+    if (log) {
+      log->bond_energy.push_back(sl.energy);
+      log->trunc_err.push_back(sl.trunc_err);
+      log->kept.push_back(sl.kept);
+      log->t_dav += sl.t_dav;
+      log->t_svd += sl.t_svd;
+      log->t_env += now_s() - te + t_rebuild;
+      log->n_matvec += sl.n_matvec;
+      log->env_elems.push_back(env_elems(direction == MoveFromLeft ? TL[site + 1] : TR[site - 1]));
+      size_t in_use = 0, peak = 0, cached = 0;
+      ttg_mem_info(&in_use, &peak, &cached);
+      log->pool_live_max = std::max(log->pool_live_max, in_use);
+    }
+  }
+  ttg_sync();
+  if (log) {
+    log->halfsweep_energy.push_back(Energy);
+    log->halfsweep_seconds.push_back(now_s() - t0);
+  }
+}
+
+// dmrg.cpp:702-755: schedules cycled per sweep (max_bd[sweep % size], ...)
 double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart, double tol,
             DmrgLog* log) {
-  const int L = (int)H.length;
-  int direction;
-  const int search_space_size = 3;
-  psi.position(0);
+  return dmrg(psi, H, 0, H.length - 1, num_sweeps, max_bd, cutoff, max_restart, tol, log);
+}
+
+// dmrg.cpp:760-813: sweeps over the bonds of sites [start, stop] with the
+// environments outside the range contracted once (buildEnv(start, stop))
+double dmrg(qMPS<double>& psi, qMPO<double>& H, unsigned start, unsigned stop, int num_sweeps,
+            const std::vector<int>& max_bd, const std::vector<double>& cutoff,
+            const std::vector<int>& max_restart, double tol, DmrgLog* log) {
+  const unsigned L = H.length;
+  if (psi.length != L || !(start < stop && stop < L)) fail("dmrg: bad site range");
+  if (max_bd.empty() || cutoff.empty() || max_restart.empty()) fail("dmrg: empty schedule");
+  psi.position((int)start);
   psi.normalize();
   std::vector<qtensor<double>> TR(L), TL(L);
-  buildEnv(psi, H, TR, TL);
-  double Energy = 0;
-  {
-    big_qtensor<double> A;
-    A.setLeft(&TL[0]);
-    A.addMid(&H.A[0]);
-    A.setRight(&TR[0]);
-    Energy = A.expec(psi.A[0]);
-    if (log) log->initial_energy = Energy;
-  }
-  if (g_progress_every > 0) {
-    fprintf(stderr, "[dmrg] canonical form and environments ready, E0=%.12f\n", Energy);
-    fflush(stderr);
-  }
+  if (start == 0 && stop == L - 1) buildEnv(psi, H, TR, TL);
+  else buildEnv(psi, H, TR, TL, start, stop);
+  double Energy = initial_energy(psi, H, TR, TL, start, log);
   for (int sweep = 0; sweep < num_sweeps; sweep++) {
     const int lbd = sweep % (int)max_bd.size();
     const int lc = sweep % (int)cutoff.size();
     const int lmr = sweep % (int)max_restart.size();
-    for (int l = 2 * sweep; l < 2 * (sweep + 1); l++) {
-      double t0 = now_s();
-      direction = (l % 2 == 0) ? MoveFromLeft : MoveFromRight;
-      for (int i = 0; i < L - 1; i++) {
-        unsigned site = (direction == MoveFromLeft) ? (unsigned)i : (unsigned)(L - 1 - i);
-        SiteLog sl{};
-        updateSite(psi, H, TR, TL, site, Energy, direction, max_bd[lbd], cutoff[lc], 'S',
-                   search_space_size, max_restart[lmr], tol, &sl);
-        double te = now_s();
-        updateEnv(psi, H, TR, TL, site, direction);
-        // The environment this bond read on the far side is stale: the next
-        // half-sweep recomputes it before any bond reads it.  Releasing it
-        // keeps one environment set on the device instead of two (results
-        // are unchanged; the open-boundary ends TL[0] and TR[L-1] are kept).
-        if (direction == MoveFromLeft) {
-          if (site + 1 < (unsigned)L - 1) TR[site + 1] = qtensor<double>();
-          if (site == 0) TR[0] = qtensor<double>();
-        } else {
-          if (site - 1 >= 1) TL[site - 1] = qtensor<double>();
-          if (site == (unsigned)L - 1) TL[L - 1] = qtensor<double>();
-        }
-        ttg_sync();
-        if (g_progress_every > 0 && (i % g_progress_every == 0 || i == L - 2)) {
-          size_t in_use = 0, peak = 0, cached = 0;
-          ttg_mem_info(&in_use, &peak, &cached);
-          fprintf(stderr, "[dmrg] sweep %d half %d bond %d/%d E=%.12f kept=%u %.2fs (device pool %.1f GB live, "
-                  "%.1f GB cached)\n", sweep, l % 2, i + 1, L - 1, sl.energy, sl.kept, now_s() - t0, in_use / 1e9,
-                  cached / 1e9);
-          fflush(stderr);
-        }
-        if (log) {
-          log->bond_energy.push_back(sl.energy);
-          log->trunc_err.push_back(sl.trunc_err);
-          log->kept.push_back(sl.kept);
-          log->t_dav += sl.t_dav;
-          log->t_svd += sl.t_svd;
-          log->t_env += now_s() - te;
-          log->n_matvec += sl.n_matvec;
-        }
-      }
-      ttg_sync();
-      if (log) {
-        log->halfsweep_energy.push_back(Energy);
-        log->halfsweep_seconds.push_back(now_s() - t0);
-      }
-    }
+    for (int l = 2 * sweep; l < 2 * (sweep + 1); l++)
+      half_sweep(psi, H, TR, TL, (l % 2 == 0) ? MoveFromLeft : MoveFromRight, start, stop, max_bd[lbd],
+                 cutoff[lc], max_restart[lmr], tol, Energy, sweep, log);
     psi.normalize();
   }
   psi.position(0);
+  return Energy;
+}
+
+// dmrg.cpp:653-697: half-sweeps start_sweep .. num_sweeps-1 (even ones move
+// from the left), one bond-dimension / cutoff / restart setting; a run
+// saved after half-sweep k resumes with start_sweep = k + 1
+double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, int max_bd, double cutoff, char mode,
+            int search_space_size, int max_restart, int start_sweep, double tol, DmrgLog* log) {
+  (void)mode; (void)search_space_size;
+  const unsigned L = H.length;
+  if (psi.length != L || L < 2) fail("dmrg: chain lengths differ");
+  if (start_sweep < 0) fail("dmrg: negative start_sweep");
+  const bool from_right = start_sweep % 2 == 1;
+  psi.position(from_right ? (int)L - 1 : 0);
+  psi.normalize();
+  std::vector<qtensor<double>> TR(L), TL(L);
+  // an even start reads TL[0] and the right environments, which buildEnv
+  // makes; an odd start moves from the right first and reads the left
+  // environments, built here from site 0 (the reference calls buildEnv for
+  // both and leaves TL[1..L-1] empty on an odd start)
+  if (!from_right) buildEnv(psi, H, TR, TL);
+  else buildEnvLeft(psi, H, TR, TL);
+  double Energy = initial_energy(psi, H, TR, TL, from_right ? L - 1 : 0, log);
+  for (int l = start_sweep; l < num_sweeps; l++)
+    half_sweep(psi, H, TR, TL, (l % 2 == 0) ? MoveFromLeft : MoveFromRight, 0, L - 1, max_bd, cutoff,
+               max_restart, tol, Energy, l / 2, log);
+  psi.position(0);
+  psi.normalize();
   return Energy;
 }
 

@@ -232,6 +232,9 @@ double qtensor<double>::normalize() {
 static int g_heff_prec = TTG_FP64;
 void set_heff_precision(int prec) { g_heff_prec = prec; }
 int heff_precision() { return g_heff_prec; }
+static int g_env_prec = TTG_FP64;
+void set_env_precision(int prec) { g_env_prec = prec; }
+int env_precision() { return g_env_prec; }
 
 big_qtensor<double>::~big_qtensor() { drop(); }
 

@@ -154,6 +154,10 @@ class qtensor<double> {
 // precision of the effective-Hamiltonian contractions (TTG_FP64 default)
 void set_heff_precision(int prec);
 int heff_precision();
+// precision of the environment updates' three contractions (TTG_FP64
+// default; the environments themselves are FP64 in every form)
+void set_env_precision(int prec);
+int env_precision();
 
 // qtensor/big_qtensor.{h,cpp}
 template <typename T>
@@ -237,6 +241,12 @@ class qTensorTrain<double, N> {
   void lc();
   double position(int site);
   void normalize();
+  // qtt.cpp:678-771: metadata to <fn>.h5, block data to wfn (default
+  // <fn>__T.bin) at cumulative byte offsets; load reads the data file named
+  // in the metadata with dataPrefix prepended.  Defined in libttgpu_io.so
+  // (tensor-tools_amd/io/checkpoint.cpp, HDF5).
+  void save(std::string fn, std::string wfn = "");
+  void load(std::string fn, std::string dataPrefix = "");
 };
 template <typename T>
 using qMPS = qTensorTrain<T, 1>;
@@ -263,6 +273,9 @@ double psiphi(qMPS<double>& psi, qMPS<double>& phi);
 // algos/dmrg/dmrg.cpp
 void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
               std::vector<qtensor<double>>& TL);
+// dmrg.cpp:89-134: environments of the site range [start, stop]
+void buildEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
+              std::vector<qtensor<double>>& TL, unsigned start, unsigned stop);
 void updateEnv(qMPS<double>& psi, qMPO<double>& H, std::vector<qtensor<double>>& TR,
                std::vector<qtensor<double>>& TL, const unsigned& site, int& direction);
 struct SiteLog { double energy, trunc_err; unsigned kept; int n_matvec; double t_dav, t_svd, t_env; };
@@ -274,13 +287,28 @@ struct DmrgLog {
   double initial_energy = 0;
   std::vector<double> halfsweep_energy, halfsweep_seconds, bond_energy, trunc_err;
   std::vector<unsigned> kept;
+  std::vector<size_t> env_elems;   // elements of the environment each bond produced
+  size_t pool_live_max = 0;        // largest device-pool live bytes after a bond
   double t_dav = 0, t_svd = 0, t_env = 0;
   long n_matvec = 0;
 };
 // progress line on stderr every `every` bonds of a half-sweep (0: silent)
 void set_dmrg_progress(int every);
+// environments kept between half-sweeps: those of every k-th site and the
+// ends; the others are rebuilt from the nearest kept one when a bond needs
+// them (k = 1, the default, keeps one full set).  Results do not depend on k.
+void set_env_checkpoint_stride(int k);
 double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart,
             double tol = 1e-12, DmrgLog* log = nullptr);
+// dmrg.cpp:760-813: bonds of sites [start, stop] only
+double dmrg(qMPS<double>& psi, qMPO<double>& H, unsigned start, unsigned stop, int num_sweeps,
+            const std::vector<int>& max_bd, const std::vector<double>& cutoff,
+            const std::vector<int>& max_restart, double tol = 1e-12, DmrgLog* log = nullptr);
+// dmrg.cpp:653-697: half-sweeps start_sweep .. num_sweeps-1 (the resume
+// form; num_sweeps counts half-sweeps)
+double dmrg(qMPS<double>& psi, qMPO<double>& H, int num_sweeps, int max_bd, double cutoff, char mode,
+            int search_space_size, int max_restart, int start_sweep, double tol = 1e-12,
+            DmrgLog* log = nullptr);
 
 }  // namespace ttgpu

@@ -4,6 +4,7 @@ tth_* entry points of libttgpu.so.  Method names follow the reference API."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -24,6 +25,7 @@ def _lib():
     if not _sig_done:
         sig = {
             "tth_init": ([_i], _i), "tth_set_precision": ([_i], None),
+            "tth_set_env_precision": ([_i], None),
             "tth_last_error": ([], C.c_char_p),
             "tth_tensor_import": ([_i, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip, _i, _ip, _dp], _p),
             "tth_init_block": ([_i, _ip, _ip, _ip, C.c_char_p, _ip, _ip, _ip], _p),
@@ -54,7 +56,13 @@ def _lib():
             "tth_mps_position": ([_p, _i], _d), "tth_mps_normalize": ([_p], _i),
             "tth_qtt_bond_dims": ([_p, _ip], None),
             "tth_dmrg": ([_p, _p, _i, _i, _ip, _i, _dp, _i, _ip, _d, _dp, _dp, C.POINTER(_d), _dp,
-                          _dp, _dp, _ip], _d),
+                          _dp, _dp, _ip, _dp], _d),
+            "tth_dmrg_range": ([_p, _p, _i, _i, _i, _i, _ip, _i, _dp, _i, _ip, _d, _dp, _dp,
+                                C.POINTER(_d), _dp, _dp, _dp, _ip, _dp], _d),
+            "tth_dmrg_resume": ([_p, _p, _i, _i, _d, _i, _i, _d, _dp, _dp, C.POINTER(_d), _dp, _dp,
+                                 _dp, _ip, _dp], _d),
+            "tth_set_env_checkpoint_stride": ([_i], None),
+            "tth_mps_new": ([], _p), "tth_mpo_new": ([], _p),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -66,6 +74,11 @@ def _lib():
 
 def set_precision(prec: int):
     _lib().tth_set_precision(prec)
+
+
+def set_env_precision(prec: int):
+    """Arithmetic of the environment updates' contractions (FP64 default)."""
+    _lib().tth_set_env_precision(prec)
 
 
 def _err(what):
@@ -217,7 +230,54 @@ def svd_bond(combined, A_left, A_right, mid_name, direction, cutoff, K):
     return tr, kept.value, PTensor(S.value)
 
 
-class PMPO:
+_io = None
+
+
+def _io_lib():
+    """libttgpu_io.so: qTensorTrain save / load (HDF5 metadata + flat block
+    data, qtt.cpp:678-771)."""
+    global _io
+    if _io is None:
+        _lib()
+        path = os.path.join(os.path.dirname(T.LIB_PATH), "libttgpu_io.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"libttgpu_io.so not built ({path}); run __graft_entry__.build()")
+        L = C.CDLL(path)
+        for name in ("tth_mps_save", "tth_mpo_save", "tth_mps_load", "tth_mpo_load"):
+            f = getattr(L, name)
+            f.argtypes = [_p, C.c_char_p, C.c_char_p]
+            f.restype = _i
+        L.tth_io_last_error.argtypes = []
+        L.tth_io_last_error.restype = C.c_char_p
+        _io = L
+    return _io
+
+
+def _io_ok(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"ttgpu io: {what}: {_io_lib().tth_io_last_error().decode()}")
+
+
+class _Train:
+    """save / load of qMPS and qMPO (qtt.cpp:678-771)"""
+    _kind = ""
+
+    def save(self, fn: str, wfn: str = ""):
+        """metadata to fn + '.h5', block data to wfn (default fn + '__T.bin')"""
+        _io_ok(getattr(_io_lib(), f"tth_{self._kind}_save")(self.h, fn.encode(), wfn.encode()), "save")
+
+    @classmethod
+    def load(cls, fn: str, data_prefix: str = ""):
+        """a train read from fn (the .h5 file); its data file name, as stored,
+        is prefixed with data_prefix"""
+        t = cls(getattr(_lib(), f"tth_{cls._kind}_new")())
+        _io_ok(getattr(_io_lib(), f"tth_{cls._kind}_load")(t.h, fn.encode(), data_prefix.encode()), "load")
+        return t
+
+
+class PMPO(_Train):
+    _kind = "mpo"
+
     def __init__(self, h):
         self.h = _check(h, "mpo")
 
@@ -232,7 +292,9 @@ class PMPO:
         return PTensor(_lib().tth_mpo_tensor(self.h, i))
 
 
-class PMPS:
+class PMPS(_Train):
+    _kind = "mps"
+
     def __init__(self, h):
         self.h = _check(h, "mps")
 
@@ -305,19 +367,60 @@ def psiphi(psi: PMPS, phi: PMPS) -> float:
     return _ok(_lib().tth_psiphi(psi.h, phi.h), "psiphi")
 
 
-def dmrg(psi: PMPS, H: PMPO, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12):
-    mb = np.ascontiguousarray(np.array(maxbd, np.int32))
-    ct = np.ascontiguousarray(np.array(cutoff, np.float64))
-    mr = np.ascontiguousarray(np.array(max_restart, np.int32))
-    hs = np.zeros(2 * nsweeps)
-    hsec = np.zeros(2 * nsweeps)
-    e0 = _d(0)
-    stats = np.zeros(7)
-    nb = 2 * nsweeps * (len(H) - 1)
-    bE, tr, kp = np.zeros(max(nb, 1)), np.zeros(max(nb, 1)), np.zeros(max(nb, 1), np.int32)
-    E = _ok(_lib().tth_dmrg(psi.h, H.h, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr, tol, hs,
-                            hsec, C.byref(e0), stats, bE, tr, kp), "dmrg")
+def _dmrg_out(E, hs, hsec, e0, stats, bE, tr, kp, env, nb):
     return dict(E=E, halfsweep_E=hs, halfsweep_s=hsec, E0=e0.value, t_davidson=stats[0],
                 t_svd=stats[1], t_env=stats[2], n_matvec=int(stats[3]), seconds=stats[4],
-                trunc_err_max=stats[5], trunc_err_last_halfsweep=stats[6], bond_E=bE[:nb],
-                trunc_err=tr[:nb], kept=kp[:nb])
+                trunc_err_max=stats[5], trunc_err_last_halfsweep=stats[6], pool_live_max=stats[7],
+                bond_E=bE[:nb], trunc_err=tr[:nb], kept=kp[:nb], env_elems=env[:nb])
+
+
+def _dmrg_bufs(nhalf, nb):
+    return (np.zeros(max(nhalf, 1)), np.zeros(max(nhalf, 1)), _d(0), np.zeros(8), np.zeros(max(nb, 1)),
+            np.zeros(max(nb, 1)), np.zeros(max(nb, 1), np.int32), np.zeros(max(nb, 1)))
+
+
+def set_env_checkpoint_stride(k: int):
+    """Keep the environments of every k-th site between half-sweeps and
+    rebuild the others when a bond needs them (1: one full set; results do
+    not depend on k)."""
+    _lib().tth_set_env_checkpoint_stride(k)
+
+
+def _schedules(maxbd, cutoff, max_restart):
+    return (np.ascontiguousarray(np.array(maxbd, np.int32)), np.ascontiguousarray(np.array(cutoff, np.float64)),
+            np.ascontiguousarray(np.array(max_restart, np.int32)))
+
+
+def dmrg(psi: PMPS, H: PMPO, nsweeps: int, maxbd, cutoff, max_restart, tol=1e-12):
+    """dmrg(qMPS, qMPO, num_sweeps, max_bd, cutoff, max_restart) (dmrg.cpp:702-755)"""
+    mb, ct, mr = _schedules(maxbd, cutoff, max_restart)
+    nb = 2 * nsweeps * (len(H) - 1)
+    hs, hsec, e0, stats, bE, tr, kp, env = _dmrg_bufs(2 * nsweeps, nb)
+    E = _ok(_lib().tth_dmrg(psi.h, H.h, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr, tol, hs,
+                            hsec, C.byref(e0), stats, bE, tr, kp, env), "dmrg")
+    return _dmrg_out(E, hs, hsec, e0, stats, bE, tr, kp, env, nb)
+
+
+def dmrg_range(psi: PMPS, H: PMPO, start: int, stop: int, nsweeps: int, maxbd, cutoff, max_restart,
+               tol=1e-12):
+    """dmrg(qMPS, qMPO, start, stop, ...) (dmrg.cpp:760-813): bonds of sites
+    [start, stop] only"""
+    mb, ct, mr = _schedules(maxbd, cutoff, max_restart)
+    nb = 2 * nsweeps * (stop - start)
+    hs, hsec, e0, stats, bE, tr, kp, env = _dmrg_bufs(2 * nsweeps, nb)
+    E = _ok(_lib().tth_dmrg_range(psi.h, H.h, start, stop, nsweeps, len(mb), mb, len(ct), ct, len(mr), mr,
+                                  tol, hs, hsec, C.byref(e0), stats, bE, tr, kp, env), "dmrg")
+    return _dmrg_out(E, hs, hsec, e0, stats, bE, tr, kp, env, nb)
+
+
+def dmrg_resume(psi: PMPS, H: PMPO, num_half: int, max_bd: int, cutoff: float, max_restart: int,
+                start_sweep: int, tol=1e-12):
+    """dmrg(qMPS, qMPO, num_sweeps, max_bd, cutoff, mode, search_space_size,
+    max_restart, start_sweep) (dmrg.cpp:653-697): half-sweeps start_sweep ..
+    num_half - 1, even ones moving from the left"""
+    nh = max(num_half - start_sweep, 0)
+    nb = nh * (len(H) - 1)
+    hs, hsec, e0, stats, bE, tr, kp, env = _dmrg_bufs(nh, nb)
+    E = _ok(_lib().tth_dmrg_resume(psi.h, H.h, num_half, max_bd, cutoff, max_restart, start_sweep, tol, hs,
+                                   hsec, C.byref(e0), stats, bE, tr, kp, env), "dmrg")
+    return _dmrg_out(E, hs, hsec, e0, stats, bE, tr, kp, env, nb)

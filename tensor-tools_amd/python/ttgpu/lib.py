@@ -77,6 +77,9 @@ def load(path: str = LIB_PATH):
         "ttg_heff_free": ([_p], _i),
         "ttg_heff_stage_time": ([_p, _p, _i, _dp], _i),
         "ttg_heff_tiles": ([_p, _ip, _ip], _i),
+        "ttg_heff_stage_plan": ([_p, _i, C.POINTER(_p)], _i),
+        "ttg_env_update": ([_p, _p, _p, _i, _i, C.POINTER(_p)], _i),
+        "ttg_env_pairs": ([_i, _ip], _i),
         "ttg_same_blocks": ([_p, _p, C.POINTER(_i)], _i),
         "ttg_svd": ([_p, _i, _i, _d, _i64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                      C.POINTER(_d), C.POINTER(_i64)], _i),
@@ -225,8 +228,14 @@ class Plan:
         check(lib().ttg_contract_plan(A.h, B.h, len(pairs), pa, pb, prec, C.byref(self.h)))
         self.cidx = result_indices(A.idx, B.idx, pairs)
 
+    @classmethod
+    def borrowed(cls, h, cidx, owner) -> "Plan":
+        p = cls.__new__(cls)
+        p.h, p.cidx, p._owner = h, cidx, owner
+        return p
+
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+        if getattr(self, "h", None) and _lib is not None and getattr(self, "_owner", None) is None:
             _lib.ttg_plan_free(self.h)
 
     def info(self):
@@ -300,11 +309,42 @@ class Heff:
         check(lib().ttg_heff_stage_time(self.h, x.h, n, out))
         return out
 
+    def stage_plan(self, stage: int, A_idx, B_idx, pairs) -> "Plan":
+        """Borrowed view of stage `stage`'s plan (freed with the operator)."""
+        h = _p()
+        check(lib().ttg_heff_stage_plan(self.h, stage, C.byref(h)))
+        return Plan.borrowed(h, result_indices(A_idx, B_idx, pairs), self)
+
     def tiles(self):
         cfg = np.zeros(3, np.int32)
         nt = np.zeros(3, np.int32)
         check(lib().ttg_heff_tiles(self.h, cfg, nt))
         return cfg, nt
+
+
+def env_pairs(direction: int):
+    """The three stages' index pairs of ttg_env_update for a direction."""
+    out = np.zeros(16, np.int32)
+    check(lib().ttg_env_pairs(direction, out))
+    res, k = [], 0
+    for _ in range(3):
+        n = int(out[k]); k += 1
+        res.append([(int(out[k + 2 * i]), int(out[k + 2 * i + 1])) for i in range(n)])
+        k += 2 * n
+    return res
+
+
+def env_update(psi: DTensor, env: DTensor, W: DTensor, direction: int, prec=FP64) -> DTensor:
+    """env' = ((psi^dag' * env) * W) * psi (updateEnv, dmrg.cpp:491-509)."""
+    h = _p()
+    check(lib().ttg_env_update(psi.h, env.h, W.h, direction, prec, C.byref(h)))
+    p = env_pairs(direction)
+    psid = [ix.dag() for ix in psi.idx]
+    for ix in psid:
+        ix.level += 1
+    i1 = result_indices(psid, env.idx, p[0])
+    i2 = result_indices(i1, W.idx, p[1])
+    return DTensor(h, result_indices(i2, psi.idx, p[2]))
 
 
 def embed(src: DTensor, dst: DTensor):

@@ -24,3 +24,10 @@ def test_binding_covers_header():
 def test_last_error_without_gpu_is_a_string():
     L = T.load()
     assert isinstance(L.ttg_last_error(), bytes)
+
+
+def test_io_library_loads_and_exports_save_load():
+    from ttgpu import host as P
+    L = P._io_lib()
+    for s in ("tth_mps_save", "tth_mps_load", "tth_mpo_save", "tth_mpo_load", "tth_io_last_error"):
+        assert hasattr(L, s), s

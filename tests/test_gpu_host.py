@@ -187,10 +187,12 @@ def test_dmrg_n20_matches_ed_and_oracle():
     st = [i % 2 for i in range(N)]
     H = P.mpo_heisenberg_nn(N)
     psi = P.mps_product(st)
-    r = P.dmrg(psi, H, 10, [16, 32, 64, 64], [1e-10], [2], 1e-10)
-    assert abs(r["E"] - ED["heisenberg_obc_N20"]) / N < 1e-9
+    r = P.dmrg(psi, H, 10, [16, 32, 48, 64, 64, 64, 64, 64, 64, 64], [1e-10], [2], 1e-10)
+    # the ramp ends at m = 64 for its last seven sweeps; the m = 64
+    # truncation leaves the oracle 1.74e-10 per site above ED
+    assert abs(r["E"] - ED["heisenberg_obc_N20"]) / N < 3e-10
     assert r["E0"] == pytest.approx(-(N - 1) / 4.0)
-    Eo, hs, _, _ = O.dmrg(O.mps_product(st), O.mpo_heisenberg_nn(N), 10, [16, 32, 64, 64], [1e-10],
+    Eo, hs, _, _ = O.dmrg(O.mps_product(st), O.mpo_heisenberg_nn(N), 10, [16, 32, 48, 64, 64, 64, 64, 64, 64, 64], [1e-10],
                           [2], 1e-10)
     assert abs(r["E"] - Eo) / N < 1e-10
-    assert np.allclose(r["halfsweep_E"][-4:], hs[-4:], atol=1e-9)
+    assert np.allclose(r["halfsweep_E"][-4:], hs[-4:], atol=1e-10)
