@@ -121,10 +121,16 @@ def test_ranged_dmrg():
     rc = P.dmrg_range(c, H, 3, 8, 2, [16], [1e-10], [2], 1e-10)
     assert len(rc["bond_E"]) == 2 * 2 * 5
     assert rc["E"] < rc["E0"] - 0.1
-    # only the bonds inside the range grew
+    # only the links inside the range grew: outside it the bond dimensions are
+    # those the canonicalisation alone gives the product state (its link
+    # sector sets, one state each)
     bd = c.bond_dims()
-    assert max(bd[4:9]) > 1, bd
-    assert all(bd[i] == 1 for i in range(1, 4)) and all(bd[i] == 1 for i in range(9, N + 1)), bd
+    q = P.mps_product([i % 2 for i in range(N)])
+    q.position(3)
+    q.position(0)
+    bq = q.bond_dims()
+    assert np.array_equal(bd[:4], bq[:4]) and np.array_equal(bd[9:], bq[9:]), (bd, bq)
+    assert max(bd[4:9]) > max(bq[4:9]), (bd, bq)
 
 
 def test_env_checkpoint_stride_gives_identical_results():
