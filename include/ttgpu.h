@@ -188,6 +188,16 @@ int ttg_lincomb(int nx, const ttg_tensor* xs, const double* c, ttg_tensor y);
  * same sector tuple (the block-appending half of qtensor::add,
  * qtensor.cpp:1141-1154): widens a tensor to a larger block list */
 int ttg_embed(ttg_tensor src, ttg_tensor dst);
+/* ttg_embed without any device allocation (no reference counterpart): for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc.  Its scratch (the
+ * sorted destination keys, the block map, the error count) is a fixed 1.75
+ * MiB device array of the library, located at ttg_init; src and dst may hold
+ * at most 65536 blocks each (larger inputs are refused with an error before
+ * any launch; use ttg_embed).  Every call on a device shares that device's
+ * array: calls that overlap on one device race, so issue them from one
+ * thread (the library's single stream orders them). */
+int ttg_embed_noalloc(ttg_tensor src, ttg_tensor dst);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,45 @@ __global__ void __launch_bounds__(512) k_sum_parts(const double* part, int n, do
   if (threadIdx.x == 0) *out = s;
 }
 
+__global__ void k_fill_i32(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- embed without allocation
+// Device scratch of ttg_embed_noalloc: the sorted destination keys and ids,
+// the source-to-destination block map and an error counter, carved from one
+// file-scope array.  Each device holds its own copy; host code reaches it
+// only through the address hipGetSymbolAddress returns for that device.
+constexpr int kNoAllocMaxBlocks = 65536;
+constexpr size_t kNoAllocBytes = (size_t)kNoAllocMaxBlocks * (8 + 8 + 4 + 4) +   // keys, skeys, ids, sids
+                                 (size_t)kNoAllocMaxBlocks * 4 + 256;               // map, err
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+struct NoAllocDev {
+  unsigned char* scratch = nullptr;   // device address of `scratch` on this device
+  int* flag = nullptr;                // pinned host word for the error count
+};
+static NoAllocDev g_noalloc[kMaxDevices];
+
+// Set-up, called by ttg_init with the device current: the device address of
+// the scratch array (each device holds its own copy) and a pinned host word
+// for reading the error count back.
+int noalloc_init() {
+  int dev = -1;
+  TTG_HIP(hipGetDevice(&dev));
+  TTG_REQUIRE(dev >= 0 && dev < kMaxDevices, "embed_noalloc: device id out of range");
+  NoAllocDev& d = g_noalloc[dev];
+  if (!d.scratch) {
+    void* p = nullptr;
+    TTG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TTG_REQUIRE(p, "embed_noalloc: no device address for the scratch array");
+    d.scratch = (unsigned char*)p;
+  }
+  if (!d.flag) TTG_HIP(hipHostMalloc((void**)&d.flag, sizeof(int), hipHostMallocDefault));
+  return 0;
+}
+
 }  // namespace ttg
 
 using namespace ttg;
@@ -201,6 +240,46 @@ int ttg_embed(ttg_tensor src, ttg_tensor dst) {
   TTG_HIP(hipStreamSynchronize(s));
   mark_written(dst);
   TTG_REQUIRE(herr == 0, "embed: source block missing from destination block list");
+  return 0;
+}
+
+int ttg_embed_noalloc(ttg_tensor src, ttg_tensor dst) {
+  TTG_REQUIRE(src && dst, "embed_noalloc: null tensor");
+  TTG_REQUIRE(src->m->same_indices(*dst->m), "embed_noalloc: index sectors differ");
+  const int ns = src->m->nblocks, nd = dst->m->nblocks;
+  TTG_REQUIRE(ns <= kNoAllocMaxBlocks && nd <= kNoAllocMaxBlocks,
+              "embed_noalloc: more than 65536 blocks in the source or the destination "
+              "(the limit of the library's fixed scratch; use ttg_embed)");
+  TTG_TRY(ensure_init());
+  int dev = -1;
+  TTG_HIP(hipGetDevice(&dev));
+  TTG_REQUIRE(dev >= 0 && dev < kMaxDevices && g_noalloc[dev].scratch && g_noalloc[dev].flag,
+              "embed_noalloc: library not initialised on this device (ttg_init)");
+  uint64_t* keys = (uint64_t*)g_noalloc[dev].scratch;
+  uint64_t* skeys = keys + kNoAllocMaxBlocks;
+  int* ids = (int*)(skeys + kNoAllocMaxBlocks);
+  int* sids = ids + kNoAllocMaxBlocks;
+  int* map = sids + kNoAllocMaxBlocks;
+  int* err = map + kNoAllocMaxBlocks;
+  hipStream_t s = stream();
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, err, 1, 0);
+  fill_f64(dst->d_data, dst->m->total, 0.0);
+  if (nd) {
+    hipLaunchKernelGGL(k_tuple_keys, dim3((nd + 255) / 256), dim3(256), 0, s, dst->m->view(), keys, ids);
+    hipLaunchKernelGGL(k_sort_keys, dim3((nd + 255) / 256), dim3(256), 0, s, keys, ids, nd, skeys, sids);
+  }
+  if (ns) {
+    hipLaunchKernelGGL(k_embed_map, dim3((ns + 255) / 256), dim3(256), 0, s, src->m->view(), skeys,
+                       sids, nd, dst->m->d_off, map, err);
+    hipLaunchKernelGGL(k_embed_copy, dim3(std::min(ns, 2048)), dim3(256), 0, s, ns, map,
+                       src->m->d_off, dst->m->d_off, src->d_data, dst->d_data);
+  }
+  TTG_HIP(hipGetLastError());
+  int* flag = g_noalloc[dev].flag;
+  TTG_HIP(hipMemcpyAsync(flag, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  TTG_HIP(hipStreamSynchronize(s));
+  mark_written(dst);
+  TTG_REQUIRE(*flag == 0, "embed_noalloc: source block missing from destination block list");
   return 0;
 }
 

@@ -137,6 +137,7 @@ int ttg_init(int device) {
   g_pool.budget = mfree > reserve ? mfree - reserve : mfree / 2;
   g_rt.device = device;
   g_rt.inited = true;
+  TTG_TRY(noalloc_init());
   return 0;
 }
 

@@ -45,6 +45,9 @@ struct Runtime {
 };
 Runtime& rt();
 int ensure_init();
+// set-up of the allocation-free entry points on the current device (device
+// address of their fixed scratch array; called by ttg_init)
+int noalloc_init();
 inline hipStream_t stream() { return rt().stream; }
 
 // caching device pool (size classes; buffers go back to the pool, reuse is

@@ -87,6 +87,7 @@ def load(path: str = LIB_PATH):
         "ttg_small_eigh": ([_i, _dp, _dp, _dp], _i),
         "ttg_lincomb": ([_i, C.POINTER(_p), _dp, _p], _i),
         "ttg_embed": ([_p, _p], _i),
+        "ttg_embed_noalloc": ([_p, _p], _i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -349,6 +350,11 @@ def env_update(psi: DTensor, env: DTensor, W: DTensor, direction: int, prec=FP64
 
 def embed(src: DTensor, dst: DTensor):
     check(lib().ttg_embed(src.h, dst.h))
+
+
+def embed_noalloc(src: DTensor, dst: DTensor):
+    """ttg_embed on the library's fixed scratch (no device allocation)."""
+    check(lib().ttg_embed_noalloc(src.h, dst.h))
 
 
 def small_eigh(M: np.ndarray):
