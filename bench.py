@@ -73,6 +73,7 @@ def sweep_leg(P, T, args):
     forms = {"fp64": T.FP64, "fp32": T.FP32, "bf16x3": T.BF16X3}
     res = {}
     P.set_dmrg_progress(12)
+    P.set_env_checkpoint_stride(args.env_stride)
     for name in args.sweep_forms.split(","):
         P.set_precision(forms[name])
         psi = P.mps_product([i % 2 for i in range(N)])
@@ -90,19 +91,27 @@ def sweep_leg(P, T, args):
                          t_svd=round(r["t_svd"], 4), t_env=round(r["t_env"], 4),
                          trunc_err_max=r["trunc_err_max"],
                          trunc_err_last_halfsweep=r["trunc_err_last_halfsweep"],
-                         max_bond=int(np.max(psi.bond_dims())))
+                         max_bond=int(np.max(psi.bond_dims())),
+                         env_gb_max=round(8e-9 * float(np.max(r["env_elems"])), 3),
+                         env_gb_last_halfsweep=round(8e-9 * float(np.sum(r["env_elems"][-(N - 1):])), 3),
+                         pool_live_gb_max=round(r["pool_live_max"] / 1e9, 3))
+        if args.env_log:
+            with open(args.env_log, "a") as f:
+                f.write(json.dumps({"form": name, "m": args.sweep_m, "workload": args.workload,
+                                    "env_elems": [int(v) for v in r["env_elems"]]}) + "\n")
         log(f"[bench] sweep {name}: {sweep_s:.3f} s (halves {res[name]['halfsweep_s']}, "
             f"davidson {r['t_davidson']:.2f} s, svd {r['t_svd']:.2f} s, env {r['t_env']:.2f} s, "
             f"{r['n_matvec']} matvecs), E/N = {r['E'] / N:.12f}, max bond {res[name]['max_bond']}")
         del psi
     P.set_precision(T.FP64)
+    P.set_env_checkpoint_stride(1)
     if "fp64" in res:
         for name, v in res.items():
             v["dE_per_site_vs_fp64"] = abs(v["E_per_site"] - res["fp64"]["E_per_site"])
             v["dtrunc_last_halfsweep_vs_fp64"] = abs(v["trunc_err_last_halfsweep"]
                                                     - res["fp64"]["trunc_err_last_halfsweep"])
     out = dict(m=args.sweep_m, sites=N, sweeps=args.sweeps, cutoff=args.sweep_cutoff, max_restart=2,
-               forms=res)
+               env_stride=args.env_stride, forms=res)
     if args.cpu_seconds > 0 and args.sweep_m == m_w and args.sweeps == 1 and args.sweep_cutoff == 1e-9:
         out["cpu_baseline"] = cpu_bond_sample(args, N)
     return out
@@ -231,6 +240,10 @@ def main():
     ap.add_argument("--sweep-forms", default="fp64,bf16x3,fp32")
     ap.add_argument("--sweeps", type=int, default=1, help="sweeps of the sweep leg (sweep_s is per sweep)")
     ap.add_argument("--sweep-cutoff", type=float, default=1e-9, help="SVD cutoff of the sweep leg")
+    ap.add_argument("--env-stride", type=int, default=0,
+                    help="keep every k-th environment between half-sweeps (0: 1 up to m=4096, else 4)")
+    ap.add_argument("--env-log", default="",
+                    help="append each sweep form's per-bond environment sizes to this JSON-lines file")
     ap.add_argument("--converge-sweeps", type=int, default=16,
                     help="most FP64 sweeps of the converged precision comparison (0 skips it)")
     ap.add_argument("--converge-tol", type=float, default=1e-10,
@@ -243,11 +256,13 @@ def main():
         print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus "
               f"{args.gpus}); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.sweep_m > 4096:
-        print("bench.py: sweep legs above m=4096 are disabled.  At m=8192 on the 8x16 cylinder the "
-              "two environment sets reached 281 GB of device memory and the HIP runtime aborted the "
-              "queue; the sweep now keeps one set, which is not yet measured at that size.",
-              file=sys.stderr)
+    if args.env_stride == 0:
+        args.env_stride = 1 if args.sweep_m <= 4096 else 4
+    if args.sweep_m > 4096 and args.env_stride < 2:
+        print("bench.py: a sweep leg above m=4096 needs --env-stride >= 2.  One full environment set "
+              "of the 8x16 cylinder at m=8192 is projected above the device's 288 GB (m=4096: 79 GB "
+              "after two sweeps, environments growing as m^2); with stride k only every k-th "
+              "environment is kept between half-sweeps.", file=sys.stderr)
         sys.exit(2)
 
     from ttgpu import host as P

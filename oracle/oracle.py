@@ -82,6 +82,8 @@ def lib():
         L.orc_heff_time.restype = _d
         L.orc_heff_time.argtypes = [_p, _p, _i]
         L.orc_threads.restype = _i
+        L.orc_blas_name.argtypes = []
+        L.orc_blas_name.restype = C.c_char_p
         L.orc_env_step.restype = _p
         L.orc_env_step.argtypes = [_p, _p, _p]
         L.orc_svd_bond.restype = _d
@@ -373,6 +375,11 @@ def dmrg_log(psi: OQTT, H: OQTT, nsweeps: int, maxbd, cutoff, max_restart, tol=1
                        C.byref(e0), C.byref(sec), bE, tr, kp)
     return dict(E=E, halfsweep_E=hs, E0=e0.value, seconds=sec.value, bond_E=bE[:nb],
                 trunc_err=tr[:nb], kept=kp[:nb])
+
+
+def blas_name() -> str:
+    """Host BLAS behind the oracle's block products."""
+    return lib().orc_blas_name().decode()
 
 
 def threads() -> int:

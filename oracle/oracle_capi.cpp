@@ -195,6 +195,9 @@ double orc_heff_time(void* b, void* x, int n) {
   return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// host BLAS behind the oracle's block products (name, for the CPU baseline record)
+const char* orc_blas_name() { return blas_name(); }
+
 int orc_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

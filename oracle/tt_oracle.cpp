@@ -14,6 +14,7 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+#include <dlfcn.h>
 #endif
 
 namespace orc {
@@ -167,6 +168,50 @@ void pair_indices(const QTensor& A, const QTensor& B,
 
 std::vector<int> g_last_tasks;   // (c, a, b, M, N, K) of the last operator*
 
+// Host BLAS for the packed block products (the reference's path ends in
+// MKL dgemm under CTF): cblas_dgemm from the MKL runtime library when it can
+// be loaded (libmkl_rt.so, sequential inside each call, since the caller runs
+// one block per OpenMP thread), else the column-axpy loop below.  The
+// library in use is reported by orc_blas_name().
+namespace {
+typedef void (*cblas_dgemm_t)(int, int, int, int, int, int, double, const double*, int, const double*, int,
+                              double, double*, int);
+struct HostBlas {
+  cblas_dgemm_t dgemm = nullptr;
+  const char* name = "none (column-axpy loop)";
+  HostBlas() {
+    const char* off = getenv("ORACLE_NO_BLAS");
+    if (off && *off && *off != '0') return;
+    for (const char* lib : {"libmkl_rt.so", "libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so"}) {
+      void* h = dlopen(lib, RTLD_NOW | RTLD_LOCAL);
+      if (!h) continue;
+      auto f = (cblas_dgemm_t)dlsym(h, "cblas_dgemm");
+      auto seq = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+      if (!f) continue;
+      if (seq) seq(1);   // MKL_THREADING_SEQUENTIAL
+      dgemm = f;
+      name = "MKL (libmkl_rt, cblas_dgemm, sequential per block)";
+      return;
+    }
+  }
+};
+HostBlas& host_blas() {
+  static HostBlas b;
+  return b;
+}
+// C (M x N) += A (M x K) * B (K x N), all column-major; false: no BLAS
+bool host_dgemm(size_t M, size_t N, size_t K, const double* A, const double* B, double* C) {
+  HostBlas& hb = host_blas();
+  if (!hb.dgemm || M == 0 || N == 0 || K == 0 || M > (1u << 30) || N > (1u << 30) || K > (1u << 30))
+    return false;
+  hb.dgemm(102 /* CblasColMajor */, 111 /* CblasNoTrans */, 111, (int)M, (int)N, (int)K, 1.0, A, (int)M, B,
+           (int)K, 1.0, C, (int)M);
+  return true;
+}
+}  // namespace
+
+const char* blas_name() { return host_blas().name; }
+
 namespace {
 struct Task { unsigned a, b; };
 // One matched block pair C += A*B.  Operands keep their stored layout
@@ -216,6 +261,7 @@ void block_gemm(double* C, const std::vector<double>& Ab, const std::vector<int>
     for (size_t m = 0; m < M; m++) Ap[m + M * k] = Ab[oAm[m] + oAk[k]];
   for (size_t n = 0; n < N; n++)
     for (size_t k = 0; k < K; k++) Bp[k + K * n] = Bb[oBk[k] + oBn[n]];
+  if (host_dgemm(M, N, K, Ap.data(), Bp.data(), C)) return;
   for (size_t n = 0; n < N; n++) {
     double* c = C + M * n;
     for (size_t k = 0; k < K; k++) {

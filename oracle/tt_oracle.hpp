@@ -9,15 +9,20 @@
 // ClarkResearchGroup/tensor-tools (parallel branch).  Blocks are dense,
 // column-major (first index fastest), exactly as the reference's CTF blocks.
 //
-// Pinning (what checks each function; see tests/test_oracle_*.py):
-//   QN / index tags / initBlock .......... brute-force enumeration + fixtures
-//   operator* (contract_tags) ............ dense numpy einsum of densified tensors
-//   vector ops (dot/add/norm) ............ dense numpy on densified tensors
-//   big_qtensor::product ................. explicit dense H_eff (numpy) + Hermiticity
-//   tensor_davidsonIT ................... numpy.linalg.eigh of dense H_eff
+// Pinning (what checks each function; tests/test_oracle.py and
+// tests/test_oracle_pin.py):
+//   QN / index tags / initBlock .......... brute-force enumeration of legal tuples
+//   operator* (contract_tags) ............ dense numpy einsum of densified tensors;
+//                                         block and task order against an
+//                                         independent statement (tests/ref_order.py)
+//   big_qtensor::product ................. explicit dense H_eff (numpy)
+//   tensor_davidsonIT ................... lowest eigenvalue of the dense H_eff
+//                                         (numpy.linalg.eigh), variational bound
 //   jacobi_eigh / DIAG .................. numpy.linalg.eigh
-//   svd / svd_bond ...................... numpy.linalg.svd singular values,
-//                                         U.S.V reconstruction, orthonormality
+//   svd / svd_bond ...................... numpy.linalg.svd singular values, U.S.V
+//                                         reconstruction, orthonormality; keep rule
+//                                         against invariants of the sector values
+//   position / rc / lc .................. state unchanged (dense), orthonormality
 //   dmrg ................................ exact diagonalisation (scipy eigsh)
 //   MPO builders ........................ dense MPO-contracted H vs. Kronecker H
 #pragma once
@@ -211,5 +216,8 @@ struct DmrgLog { std::vector<double> halfsweep_energy; std::vector<double> bond_
 double dmrg(QTT& psi, QTT& H, int num_sweeps, const std::vector<int>& max_bd,
             const std::vector<double>& cutoff, const std::vector<int>& max_restart,
             double tol, DmrgLog* log);
+
+// host BLAS library the block products use ("none ..." for the built-in loop)
+const char* blas_name();
 
 }  // namespace orc

@@ -162,6 +162,7 @@ using namespace ttg;
 extern "C" {
 
 int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out) {
+  PhaseTimer _phase("ttg_dot_matched");
   TTG_REQUIRE(x && y && out, "dot_matched: null argument");
   const TMeta& a = *x->m;
   const TMeta& b = *y->m;
@@ -197,6 +198,7 @@ int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out) {
 }
 
 int ttg_small_eigh(int n, const double* M, double* evals, double* evecs) {
+  PhaseTimer _phase("ttg_small_eigh");
   TTG_TRY(ensure_init());
   TTG_REQUIRE(n >= 1 && n <= 16, "small_eigh: 1 <= n <= 16");
   Scratch sc;
@@ -211,6 +213,7 @@ int ttg_small_eigh(int n, const double* M, double* evals, double* evecs) {
 }
 
 int ttg_embed(ttg_tensor src, ttg_tensor dst) {
+  PhaseTimer _phase("ttg_embed");
   TTG_REQUIRE(src && dst, "embed: null tensor");
   TTG_REQUIRE(src->m->same_indices(*dst->m), "embed: index sectors differ");
   hipStream_t s = stream();

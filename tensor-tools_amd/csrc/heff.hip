@@ -77,6 +77,7 @@ __global__ void k_remap(MetaView c, const uint64_t* xkeys, const int* xids, int 
 }
 
 static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
+  PhaseTimer _phase("remap_to_x");
   hipStream_t s = stream();
   const int nx = xm->nblocks, nc = p3->C->nblocks;
   Scratch sc;
@@ -122,6 +123,7 @@ constexpr double kTuneMinFlops = 2e8;
 
 int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, bool out_f64,
                hipStream_t s) {
+  PhaseTimer _phase("tune_stage");
   if (p->cfg == TILE_SKINNY || p->ntasks == 0 || p->flops < kTuneMinFlops) return 0;
   const char* e = getenv("TTG_TILE_CFG");
   if (e && *e) return 0;
@@ -190,6 +192,7 @@ extern "C" {
 
 int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_like,
                     const int* pairs3, int prec, ttg_heff* out) {
+  PhaseTimer _phase("ttg_heff_create");
   TTG_REQUIRE(L && W12 && R && x_like && out, "heff_create: null argument");
   TTG_REQUIRE(prec == TTG_FP64 || prec == TTG_FP32 || prec == TTG_BF16X3, "unknown precision");
   ttg_heff h = new ttg_heff_s();
@@ -256,7 +259,10 @@ int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_lik
     return 1;
   }
   h->graph = g;
-  TTG_HIP(hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0));
+  {
+    PhaseTimer _inst("heff_create: graph instantiate");
+    TTG_HIP(hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0));
+  }
   *out = h;
   return 0;
 }
@@ -267,6 +273,7 @@ int ttg_heff_new_output(ttg_heff h, ttg_tensor* y) {
 }
 
 int ttg_heff_apply(ttg_heff h, ttg_tensor x, ttg_tensor y) {
+  PhaseTimer _phase("ttg_heff_apply");
   TTG_REQUIRE(h && x && y, "heff_apply: null argument");
   TTG_REQUIRE(same_meta(x->m, h->xm) && same_meta(y->m, h->xm),
               "heff_apply: x/y do not have the operator's block list");

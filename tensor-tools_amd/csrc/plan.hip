@@ -619,6 +619,7 @@ static PlanHdr* pinned_hdr() {
 
 int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
                int force_cfg, ttg_plan* out) {
+  PhaseTimer _phase("plan_build");
   TTG_TRY(ensure_init());
   TTG_REQUIRE(np >= 0 && np <= A->rank && np <= B->rank, "plan: bad pair count");
   PairSpec ps{};
@@ -819,6 +820,7 @@ int ttg_plan_free(ttg_plan p) {
 
 int ttg_contract(ttg_tensor A, ttg_tensor B, int npairs, const int* pa, const int* pb, int prec,
                  ttg_tensor* C) {
+  PhaseTimer _phase("ttg_contract");
   ttg_plan p = nullptr;
   TTG_TRY(ttg_contract_plan(A, B, npairs, pa, pb, prec, &p));
   int r = ttg_plan_new_output(p, C);

@@ -1,5 +1,9 @@
-// runtime.hip -- device binding, stream, error string and caching pool.
+// runtime.hip -- device binding, stream, error string, caching pool and the
+// host-side phase timers.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 
@@ -110,6 +114,62 @@ int Scratch::ok(const char* what) const {
 int ensure_init() {
   if (g_rt.inited) return 0;
   return ttg_init(0);
+}
+
+int side_fork(hipStream_t* side) {
+  TTG_TRY(ensure_init());
+  if (!g_rt.side) {
+    TTG_HIP(hipStreamCreateWithFlags(&g_rt.side, hipStreamNonBlocking));
+    TTG_HIP(hipEventCreateWithFlags(&g_rt.fork, hipEventDisableTiming));
+    TTG_HIP(hipEventCreateWithFlags(&g_rt.join, hipEventDisableTiming));
+  }
+  TTG_HIP(hipEventRecord(g_rt.fork, g_rt.stream));
+  TTG_HIP(hipStreamWaitEvent(g_rt.side, g_rt.fork, 0));
+  *side = g_rt.side;
+  return 0;
+}
+
+int side_join() {
+  if (!g_rt.side) return 0;
+  TTG_HIP(hipEventRecord(g_rt.join, g_rt.side));
+  TTG_HIP(hipStreamWaitEvent(g_rt.stream, g_rt.join, 0));
+  return 0;
+}
+
+// ---------------------------------------------------------------- phase timers
+// Wall-clock seconds and counts per named host phase, when the environment
+// variable TTG_HOST_PROF is set; printed to stderr at process exit.  A phase
+// covers the host code of a library call, its synchronisations included.
+namespace {
+struct PhaseAcc {
+  std::map<std::string, std::pair<double, long>> acc;
+  bool on = false;
+  PhaseAcc() {
+    const char* e = getenv("TTG_HOST_PROF");
+    on = e && *e && *e != '0';
+  }
+  ~PhaseAcc() {
+    if (!on || acc.empty()) return;
+    fprintf(stderr, "[ttg host phases] seconds  calls  phase\n");
+    for (auto& kv : acc)
+      fprintf(stderr, "[ttg host phases] %8.4f %6ld  %s\n", kv.second.first, kv.second.second, kv.first.c_str());
+  }
+};
+PhaseAcc& phases() {
+  static PhaseAcc p;
+  return p;
+}
+}  // namespace
+
+PhaseTimer::PhaseTimer(const char* name) : name_(name), on_(phases().on) {
+  if (on_) t0_ = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+PhaseTimer::~PhaseTimer() {
+  if (!on_) return;
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  auto& a = phases().acc[name_];
+  a.first += t - t0_;
+  a.second += 1;
 }
 
 }  // namespace ttg

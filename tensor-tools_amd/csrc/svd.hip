@@ -760,6 +760,7 @@ __global__ void k_bjac_finish(const BJSector* sec, int nsec, const int64_t* goff
 static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq, const int64_t* d_goff,
                         const int64_t* d_eoff, const double* G, double* Vev, double* ev,
                         hipStream_t s, int max_sweeps, double floor_rel) {
+  PhaseTimer _phase("block_jacobi");
   const int ns = (int)big.size();
   std::vector<BJSector> sec(ns);
   std::vector<int2> pitems;
@@ -1010,42 +1011,54 @@ __global__ void __launch_bounds__(256) k_svd_finite(int nQ, const int* nq, const
     if (!isfinite(G[goff[q] + i + (int64_t)n * i])) *bad = 1;
 }
 
-constexpr int kSelMax = 8192;   // largest sector (ttg_svd rejects larger ones)
+// 5a. per sector: the rank of each eigenvalue (descending, ties by index; a
+// NaN ranks as -inf, so the ranks are a permutation for any input), the kept
+// singular values in rank order and the eigenpair index of each kept rank.
+// One thread per eigenvalue (grid: sectors x 256-value groups); the sector's
+// eigenvalues pass through LDS in chunks, so every comparison reads one
+// broadcast LDS value.
+constexpr int kRankChunk = 2048;
 __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq,
                                                     const int64_t* eoff, const double* evals,
-                                                    const int64_t* goff, const double* V,
                                                     const int64_t* soff, double* svals,
-                                                    int* ssec, const int64_t* selo,
-                                                    double* Esel) {
-  __shared__ int sel[kSelMax];   // sel[r]: the eigenpair of rank r < k
+                                                    int* ssec, int* selidx) {
+  __shared__ double chunk[kRankChunk];
   const int q = blockIdx.x;
   const int n = nq[q], k = kq[q];
+  if ((int)blockIdx.y * 256 >= n) return;   // the whole workgroup: before any barrier
+  const int i = blockIdx.y * 256 + threadIdx.x;
   const double* ev = evals + eoff[q];
-  // rank of each eigenvalue (descending, ties by index): rank < k selected.
-  // The ranks are a permutation for any input (a NaN ranks as -inf), so the
-  // slots below are each written once.
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = rank_key(ev[i]);
-    int r = 0;
-    for (int j = 0; j < n; j++) {
-      const double u = rank_key(ev[j]);
-      r += (u > v) || (u == v && j < i);
-    }
-    if (r < k) {
-      svals[soff[q] + r] = sqrt(fmax(v, 0.0));
-      ssec[soff[q] + r] = q;
-      sel[r] = i;
+  const double v = i < n ? rank_key(ev[i]) : 0.0;
+  int r = 0;
+  for (int c0 = 0; c0 < n; c0 += kRankChunk) {
+    const int cn = min(kRankChunk, n - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < cn; e += 256) chunk[e] = rank_key(ev[c0 + e]);
+    __syncthreads();
+    for (int j = 0; j < cn; j++) {
+      const double u = chunk[j];
+      r += (u > v) || (u == v && c0 + j < i);
     }
   }
-  __syncthreads();
-  // the selected eigenvectors, packed in rank order: consecutive threads
-  // copy consecutive rows of a column
-  const double* src = V + goff[q];
-  double* dst = Esel + selo[q];
-  const int64_t tot = (int64_t)n * k;
-  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int r = (int)(e / n), row = (int)(e - (int64_t)r * n);
-    dst[e] = src[row + (int64_t)n * sel[r]];
+  if (i < n && r < k) {
+    svals[soff[q] + r] = sqrt(fmax(v, 0.0));
+    ssec[soff[q] + r] = q;
+    selidx[soff[q] + r] = i;
+  }
+}
+
+// the selected eigenvectors packed in rank order, Esel_q(:, r) = V_q(:, selidx_q[r]):
+// workgroups over (sector, columns), consecutive threads on consecutive rows
+__global__ void __launch_bounds__(256) k_svd_gather_sel(const int* nq, const int* kq,
+                                                        const int64_t* goff, const double* V,
+                                                        const int64_t* soff, const int* selidx,
+                                                        const int64_t* selo, double* Esel) {
+  const int q = blockIdx.x;
+  const int n = nq[q], k = kq[q];
+  for (int r = blockIdx.y; r < k; r += gridDim.y) {
+    const double* src = V + goff[q] + (int64_t)n * selidx[soff[q] + r];
+    double* dst = Esel + selo[q] + (int64_t)n * r;
+    for (int row = threadIdx.x; row < n; row += blockDim.x) dst[row] = src[row];
   }
 }
 
@@ -1056,58 +1069,77 @@ __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq
 // a stable sort as the reference's pairCompare sort over (sector, desc) order)
 __global__ void __launch_bounds__(256) k_svd_rank(int T, const double* __restrict__ svals,
                                                   int* __restrict__ order, double* __restrict__ sorted) {
+  __shared__ double chunk[kRankChunk];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= T) return;
-  const double v = rank_key(svals[i]);
+  const double v = i < T ? rank_key(svals[i]) : 0.0;
   int r = 0;
-  for (int j = 0; j < T; j++) {
-    const double u = rank_key(svals[j]);
-    r += (u > v) || (u == v && j < i);
+  for (int c0 = 0; c0 < T; c0 += kRankChunk) {   // values through LDS, as in k_svd_select
+    const int cn = min(kRankChunk, T - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < cn; e += blockDim.x) chunk[e] = rank_key(svals[c0 + e]);
+    __syncthreads();
+    for (int j = 0; j < cn; j++) {
+      const double u = chunk[j];
+      r += (u > v) || (u == v && c0 + j < i);
+    }
   }
-  order[r] = i;
-  sorted[r] = svals[i];
+  if (i < T) {
+    order[r] = i;
+    sorted[r] = svals[i];
+  }
 }
 
 // 5c. the reference's keep loop (qtensor_op.cpp:1294-1300), sequential in its
 // summation order: nm over (sector, descending) order, cumsum over the sorted
 // values
 constexpr int kTrChunk = 2048;   // values staged in LDS per pass of the serial loops
+// The serial loops add precomputed terms: every thread forms its values'
+// square s*s (for nm) and s*s/nm (for cumsum), each rounded as the
+// reference's expressions round them, and one thread adds them in order, so
+// the sums are the reference's and the serial chain holds no multiply or
+// divide.
 __global__ void __launch_bounds__(256) k_svd_truncate(int T, int nQ, const double* svals, const int* ssec,
                                                       const int* order, const double* sorted, double cutoff,
                                                       int64_t K, int* newdim, double* out) {
-  __shared__ double buf[kTrChunk];
-  __shared__ double s_cum;
+  __shared__ double val[kTrChunk];
+  __shared__ double term[kTrChunk];
+  __shared__ double s_nm, s_cum;
   __shared__ long long s_acc;
   __shared__ int s_done;
   const int tid = threadIdx.x;
-  // nm: sequential sum in the (sector, descending) order.  The workgroup
-  // stages each chunk in LDS so that thread 0's serial loop reads no global
-  // memory (a load per iteration made the loop latency-bound).
+  // nm: the sum of squares in the (sector, descending) order
   double nm = 0.0;
   for (int c0 = 0; c0 < T; c0 += kTrChunk) {
     const int cn = min(kTrChunk, T - c0);
-    for (int e = tid; e < cn; e += blockDim.x) buf[e] = svals[c0 + e];
+    for (int e = tid; e < cn; e += blockDim.x) {
+      const double v = svals[c0 + e];
+      term[e] = v * v;
+    }
     __syncthreads();
     if (tid == 0)
-      for (int e = 0; e < cn; e++) nm += buf[e] * buf[e];
+      for (int e = 0; e < cn; e++) nm += term[e];
     __syncthreads();
   }
-  if (tid == 0) { s_cum = 0.0; s_acc = 0; s_done = 0; }
+  if (tid == 0) { s_nm = nm; s_cum = 0.0; s_acc = 0; s_done = 0; }
   __syncthreads();
+  nm = s_nm;
   // the keep loop over the sorted values, in order, chunk by chunk
   for (int c0 = 0; c0 < T; c0 += kTrChunk) {
     if (s_done) break;   // read after a barrier: the same value in every thread
     const int cn = min(kTrChunk, T - c0);
-    for (int e = tid; e < cn; e += blockDim.x) buf[e] = sorted[c0 + e];
+    for (int e = tid; e < cn; e += blockDim.x) {
+      const double v = sorted[c0 + e];
+      val[e] = v;
+      term[e] = v * v / nm;
+    }
     __syncthreads();
     if (tid == 0) {
       double cumsum = s_cum;
       long long acc = s_acc;
       int done = 0;
       for (int e = 0; e < cn; e++) {
-        const double v = buf[e];
-        cumsum += v * v / nm;
-        if (v == 0. && cutoff != 0.) { done = 1; break; }
+        cumsum += term[e];
+        if (val[e] == 0. && cutoff != 0.) { done = 1; break; }
         acc += 1;
         if ((1 - cumsum < cutoff && cutoff != 0.) || (acc >= K && K != 0)) { done = 1; break; }
       }
@@ -1199,45 +1231,72 @@ __global__ void __launch_bounds__(1024) k_svd_emit(MetaView A, int nleft, SvdTab
 
 // 6b. slice sector matrices into blocks.  U_q (rows x k, ld rows) ->
 // U blocks (lsize x k);  W_q (k x cols, ld k) -> V blocks (k x rsize).
-__global__ void k_svd_scatter(SvdTab t, int nQ, const int* newdim, const int* ublk0,
-                              const int* vblk0, const int64_t* uoffs, const int64_t* voffs,
-                              const double* Umat, const int64_t* umo, const double* Wmat,
-                              const int64_t* wmo, double* Ud, double* Vd) {
+// Workgroups over (sector, columns): a column of a block is contiguous on
+// both sides.
+__global__ void __launch_bounds__(256) k_svd_scatter(SvdTab t, int nQ, const int* newdim,
+                                                     const int* ublk0, const int* vblk0,
+                                                     const int64_t* uoffs, const int64_t* voffs,
+                                                     const double* Umat, const int64_t* umo,
+                                                     const double* Wmat, const int64_t* wmo,
+                                                     double* Ud, double* Vd) {
   const int q = blockIdx.x;
   const int k = newdim[q];
   if (k == 0) return;
-  const int64_t rows = t.rows[q], cols = t.cols[q];
+  const int64_t rows = t.rows[q];
   int ub = ublk0[q];
   for (int i = t.qL0[q]; i < t.qL1[q]; i++, ub++) {
-    const int64_t ls = t.lsz[i], lo = t.loff[i];
-    double* dst = Ud + uoffs[ub];
-    for (int64_t e = threadIdx.x; e < ls * k; e += blockDim.x) {
-      const int64_t r = e % ls, c = e / ls;
-      dst[e] = Umat[umo[q] + lo + r + rows * c];
+    const int ls = (int)t.lsz[i];
+    const int64_t lo = t.loff[i];
+    for (int c = blockIdx.y; c < k; c += gridDim.y) {
+      double* dst = Ud + uoffs[ub] + (int64_t)ls * c;
+      const double* src = Umat + umo[q] + lo + rows * c;
+      for (int r = threadIdx.x; r < ls; r += blockDim.x) dst[r] = src[r];
     }
   }
   int vb = vblk0[q];
   for (int i = t.qR0[q]; i < t.qR1[q]; i++, vb++) {
-    const int64_t rs = t.rsz[i], co = t.roff[i];
-    double* dst = Vd + voffs[vb];
-    for (int64_t e = threadIdx.x; e < rs * k; e += blockDim.x) {
-      const int64_t r = e % k, c = e / k;
-      dst[e] = Wmat[wmo[q] + r + (int64_t)k * (co + c)];
+    const int rs = (int)t.rsz[i];
+    const int64_t co = t.roff[i];
+    for (int c = blockIdx.y; c < rs; c += gridDim.y) {
+      double* dst = Vd + voffs[vb] + (int64_t)k * c;
+      const double* src = Wmat + wmo[q] + (int64_t)k * (co + c);
+      for (int r = threadIdx.x; r < k; r += blockDim.x) dst[r] = src[r];
     }
   }
-  (void)cols;
 }
 
-// transpose-free copies for the unabsorbed factor: U_q = E (rows x k) for
-// MoveFromLeft; W_q = E^T (k x cols) for MoveFromRight
-__global__ void k_svd_transpose(int nQ, const int* nq, const int* newdim, const int64_t* selo,
-                                const double* E, const int64_t* wmo, double* W) {
+// the unabsorbed factor without a product: W_q = E^T (k x cols) for
+// MoveFromRight, by 32 x 32 tiles through LDS (reads along E's columns,
+// writes along W's columns); tiles over (sector, tile index)
+constexpr int kTT = 32;
+__global__ void __launch_bounds__(256) k_svd_transpose(int nQ, const int* nq, const int* newdim,
+                                                       const int64_t* selo, const double* E,
+                                                       const int64_t* wmo, double* W) {
+  __shared__ double tile[kTT][kTT + 1];
   const int q = blockIdx.x;
   const int n = nq[q], k = newdim[q];
-  for (int64_t e = threadIdx.x; e < (int64_t)n * k; e += blockDim.x) {
-    const int64_t i = e % n, c = e / n;   // E(i, c)
-    W[wmo[q] + c + (int64_t)k * i] = E[selo[q] + e];
+  const int ti = (n + kTT - 1) / kTT, tc = (k + kTT - 1) / kTT;
+  const int x = threadIdx.x % kTT, y0 = threadIdx.x / kTT;   // 32 x 8 threads
+  for (int tt = blockIdx.y; tt < ti * tc; tt += gridDim.y) {
+    const int i0 = (tt % ti) * kTT, c0 = (tt / ti) * kTT;
+    __syncthreads();
+    for (int y = y0; y < kTT; y += 8)   // E(i0 + x, c0 + y)
+      if (i0 + x < n && c0 + y < k) tile[y][x] = E[selo[q] + (i0 + x) + (int64_t)n * (c0 + y)];
+    __syncthreads();
+    for (int y = y0; y < kTT; y += 8)   // W(c0 + x, i0 + y) = E(i0 + y, c0 + x)
+      if (c0 + x < k && i0 + y < n) W[wmo[q] + (c0 + x) + (int64_t)k * (i0 + y)] = tile[x][y];
   }
+}
+
+// U_q = the first k columns of Esel_q (rows x kq, ld rows) for MoveFromLeft
+__global__ void __launch_bounds__(256) k_svd_copy_sel(int nQ, const int64_t* rowsq, const int* newdim,
+                                                      const int64_t* selo, const double* E,
+                                                      const int64_t* umo, double* U) {
+  const int q = blockIdx.x;
+  const int64_t tot = rowsq[q] * newdim[q];
+  for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.y * blockDim.x)
+    U[umo[q] + e] = E[selo[q] + e];
 }
 
 __global__ void k_svd_svals(int nQ, const int* newdim, const int64_t* soff, const double* svals,
@@ -1313,6 +1372,7 @@ static T* up(Scratch& sc, const std::vector<T>& v) {
 static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, const std::vector<int64_t>& goff,
                     const std::vector<int64_t>& eoff, const int64_t* d_eoff, int64_t gtotal, double* G,
                     double* Vev, double* ev, hipStream_t s) {
+  PhaseTimer _phase("lr_eigen");
   const int ns = (int)big.size();
   std::vector<PcSector> ps(ns);
   int nmax = 0;
@@ -1418,6 +1478,7 @@ using namespace ttg;
 extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, int64_t K,
                        ttg_tensor* Uout, ttg_tensor* Sout, ttg_tensor* Vout, double* trunc_err,
                        int64_t* kept) {
+  PhaseTimer _phase("ttg_svd");
   TTG_REQUIRE(A && Uout && Sout && Vout, "svd: null argument");
   const TMeta& m = *A->m;
   TTG_REQUIRE(nleft >= 1 && nleft < m.rank, "svd: need at least one left and one right index");
@@ -1489,6 +1550,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   double* gs = sc.get<double>(poff[nQ]);
   double* ev = sc.get<double>(eoff[nQ]);
   double* Esel = sc.get<double>(selo[nQ]);
+  int* selidx = sc.get<int>(soff[nQ]);
   double* svals = sc.get<double>(soff[nQ]);
   int* ssec = sc.get<int>(soff[nQ]);
   int* order = sc.get<int>(soff[nQ]);
@@ -1543,24 +1605,48 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     lds_need = std::max(lds_need, jac_layout(nq[q]).total);
     nsmall = std::max(nsmall, nq[q]);
   }
+  // The small sectors' one-workgroup Jacobi runs on the side stream, beside
+  // the large sectors' solver on the main stream: the two read and write
+  // disjoint sector ranges of G, the packed triangles, Vev and ev.
+  bool forked = false;
   if (nsmall > 0) {
     const size_t lds_bytes = sizeof(double) * (size_t)std::min<int64_t>(lds_need, kLdsDoubles);
     TTG_HIP(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(double) * kLdsDoubles)));
     // small sectors: fewer waves per workgroup (cheaper barriers per round)
     const int jac_threads = nsmall <= 96 ? 256 : kJacThreads;
-    hipLaunchKernelGGL(k_jacobi, dim3(nQ), dim3(jac_threads), lds_bytes, s, d_nq, d_goff, G, gs,
+    hipStream_t js = s;
+    if (!big.empty()) {
+      TTG_TRY(side_fork(&js));
+      forked = true;
+    }
+    hipLaunchKernelGGL(k_jacobi, dim3(nQ), dim3(jac_threads), lds_bytes, js, d_nq, d_goff, G, gs,
                        d_poff, Vev, ev, d_eoff, 40, kJacBlockMin, kJacFloor);
     TTG_HIP(hipGetLastError());
   }
   // large sectors: the LR step when the truncation drops zero singular
   // values (cutoff > 0), else block Jacobi on G
+  int brc = 0;
   if (!big.empty())
-    TTG_TRY(cutoff > 0 ? lr_eigen(big, nq, goff, eoff, d_eoff, goff[nQ], G, Vev, ev, s)
-                       : block_jacobi(big, nq, d_goff, d_eoff, G, Vev, ev, s, 40, kJacFloor));
+    brc = cutoff > 0 ? lr_eigen(big, nq, goff, eoff, d_eoff, goff[nQ], G, Vev, ev, s)
+                     : block_jacobi(big, nq, d_goff, d_eoff, G, Vev, ev, s, 40, kJacFloor);
+  if (forked) {
+    // join before anything else, the error path included: the scratch the
+    // side stream reads goes back to the pool when this call returns
+    const int jrc = side_join();
+    if (brc || jrc) (void)hipStreamSynchronize(s);
+    TTG_TRY(jrc);
+  }
+  TTG_TRY(brc);
   // ---- 5. selection and truncation
-  hipLaunchKernelGGL(k_svd_select, dim3(nQ), dim3(256), 0, s, d_nq, d_kq, d_eoff, ev, d_goff, Vev,
-                     d_soff, svals, ssec, d_selo, Esel);
+  {
+    int nmax = 1, kmax = 1;
+    for (int q = 0; q < nQ; q++) { nmax = std::max(nmax, nq[q]); kmax = std::max(kmax, kq[q]); }
+    hipLaunchKernelGGL(k_svd_select, dim3(nQ, (nmax + 255) / 256), dim3(256), 0, s, d_nq, d_kq, d_eoff, ev,
+                       d_soff, svals, ssec, selidx);
+    hipLaunchKernelGGL(k_svd_gather_sel, dim3(nQ, std::min(kmax, 64)), dim3(256), 0, s, d_nq, d_kq, d_goff,
+                       Vev, d_soff, selidx, d_selo, Esel);
+  }
   TTG_HIP(hipMemsetAsync(newdim, 0, sizeof(int) * nQ, s));
   {
     const int T = (int)soff[nQ];
@@ -1674,15 +1760,13 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     TTG_TRY(fromLeft ? run_host_gemm(hg, Esel, mat, Wmat) : run_host_gemm(hg, mat, Esel, Umat));
   }
   if (fromLeft) {   // U_q = E restricted to k columns: Esel already (n = rows) x k, ld rows
-    for (int q = 0; q < nQ; q++)
-      if (nd[q] > 0)
-        TTG_HIP(hipMemcpyAsync(Umat + umo[q], Esel + selo[q], sizeof(double) * rows[q] * nd[q],
-                               hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_svd_copy_sel, dim3(nQ, 32), dim3(256), 0, s, nQ, t.rows, newdim, d_selo, Esel,
+                       d_umo, Umat);
   } else {
-    hipLaunchKernelGGL(k_svd_transpose, dim3(nQ), dim3(256), 0, s, nQ, d_nq, newdim, d_selo, Esel,
+    hipLaunchKernelGGL(k_svd_transpose, dim3(nQ, 64), dim3(256), 0, s, nQ, d_nq, newdim, d_selo, Esel,
                        d_wmo, Wmat);
   }
-  hipLaunchKernelGGL(k_svd_scatter, dim3(nQ), dim3(256), 0, s, t, nQ, newdim, d_ub0, d_vb0, Um->d_off,
+  hipLaunchKernelGGL(k_svd_scatter, dim3(nQ, 64), dim3(256), 0, s, t, nQ, newdim, d_ub0, d_vb0, Um->d_off,
                      Vm->d_off, Umat, d_umo, Wmat, d_wmo, U->d_data, V->d_data);
   TTG_HIP(hipStreamSynchronize(s));
   *Uout = Uown.release();

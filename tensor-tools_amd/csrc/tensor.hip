@@ -342,6 +342,7 @@ extern "C" {
 
 int ttg_tensor_create(int rank, const int* arrows, const int* nsec, const int* qn,
                       const int* qdim, int nblocks, const int* block_qi, ttg_tensor* out) {
+  PhaseTimer _phase("ttg_tensor_create");
   MetaPtr m;
   TTG_TRY(meta_shell(rank, arrows, nsec, qn, qdim, &m));
   TTG_REQUIRE(nblocks >= 0, "nblocks < 0");
@@ -369,6 +370,7 @@ int ttg_tensor_create(int rank, const int* arrows, const int* nsec, const int* q
 
 int ttg_tensor_init_blocks(int rank, const int* arrows, const int* nsec, const int* qn,
                            const int* qdim, ttg_tensor* out) {
+  PhaseTimer _phase("ttg_tensor_init_blocks");
   MetaPtr m;
   TTG_TRY(meta_shell(rank, arrows, nsec, qn, qdim, &m));
   int64_t ncomb = 1;
@@ -392,6 +394,7 @@ int ttg_tensor_init_blocks(int rank, const int* arrows, const int* nsec, const i
 }
 
 int ttg_tensor_upload(ttg_tensor t, const double* host, size_t n) {
+  PhaseTimer _phase("ttg_tensor_upload");
   TTG_REQUIRE(t && n == t->m->total, "upload: size mismatch");
   if (n) TTG_HIP(hipMemcpyAsync(t->d_data, host, n * sizeof(double), hipMemcpyHostToDevice, stream()));
   TTG_HIP(hipStreamSynchronize(stream()));
@@ -400,6 +403,7 @@ int ttg_tensor_upload(ttg_tensor t, const double* host, size_t n) {
 }
 
 int ttg_tensor_download(ttg_tensor t, double* host, size_t n) {
+  PhaseTimer _phase("ttg_tensor_download");
   TTG_REQUIRE(t && n == t->m->total, "download: size mismatch");
   if (n) TTG_HIP(hipMemcpyAsync(host, t->d_data, n * sizeof(double), hipMemcpyDeviceToHost, stream()));
   TTG_HIP(hipStreamSynchronize(stream()));
@@ -439,6 +443,7 @@ int ttg_tensor_index_download(ttg_tensor t, int j, int* nsec, int* qn, int* qdim
 }
 
 int ttg_tensor_dag(ttg_tensor t) {
+  PhaseTimer _phase("ttg_tensor_dag");
   TTG_REQUIRE(t, "null tensor");
   MetaPtr m;
   TTG_TRY(meta_dag(*t->m, &m));
@@ -453,6 +458,7 @@ int ttg_same_blocks(ttg_tensor a, ttg_tensor b, int* same) {
 }
 
 int ttg_tensor_copy(ttg_tensor src, ttg_tensor* out) {
+  PhaseTimer _phase("ttg_tensor_copy");
   TTG_REQUIRE(src, "null tensor");
   TTG_TRY(tensor_from_meta(src->m, false, out));
   if (src->m->total)
@@ -494,6 +500,7 @@ static int read_scalar(double* out) {
 }
 
 int ttg_dot(ttg_tensor x, ttg_tensor y, double* out) {
+  PhaseTimer _phase("ttg_dot");
   TTG_REQUIRE(x && y, "null tensor");
   TTG_REQUIRE(same_structure(x, y), "dot: tensors have different block lists");
   if (!g_scalar) g_scalar = dalloc<double>(1);
@@ -503,6 +510,7 @@ int ttg_dot(ttg_tensor x, ttg_tensor y, double* out) {
 }
 
 int ttg_nrm2(ttg_tensor x, double* out) {
+  PhaseTimer _phase("ttg_nrm2");
   TTG_REQUIRE(x, "null tensor");
   if (!g_scalar) g_scalar = dalloc<double>(1);
   TTG_REQUIRE(g_scalar, "scalar allocation failed");
@@ -531,6 +539,7 @@ int ttg_axpy(double a, ttg_tensor x, ttg_tensor y) {
 }
 
 int ttg_lincomb(int nx, const ttg_tensor* xs, const double* c, ttg_tensor y) {
+  PhaseTimer _phase("ttg_lincomb");
   TTG_REQUIRE(nx >= 1 && nx <= 8, "lincomb: 1..8 operands");
   LinArgs a;
   a.n = nx;

@@ -40,15 +40,37 @@ void set_error(const std::string& msg);
 struct Runtime {
   int device = -1;
   hipStream_t stream = nullptr;
+  // a second stream (default priority, like the first) for independent work
+  // that overlaps the main stream's, joined back through events
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   int num_cu = 256;
   bool inited = false;
 };
 Runtime& rt();
 int ensure_init();
+// wall time of a host phase, accumulated under `name` when TTG_HOST_PROF is
+// set (scope-bound; the name must outlive the program, e.g. a literal)
+class PhaseTimer {
+ public:
+  explicit PhaseTimer(const char* name);
+  ~PhaseTimer();
+  PhaseTimer(const PhaseTimer&) = delete;
+  PhaseTimer& operator=(const PhaseTimer&) = delete;
+
+ private:
+  const char* name_;
+  bool on_;
+  double t0_ = 0;
+};
 // set-up of the allocation-free entry points on the current device (device
 // address of their fixed scratch array; called by ttg_init)
 int noalloc_init();
 inline hipStream_t stream() { return rt().stream; }
+// the side stream, after every piece of work already queued on the main one
+int side_fork(hipStream_t* side);
+// the main stream waits for everything queued on the side stream so far
+int side_join();
 
 // caching device pool (size classes; buffers go back to the pool, reuse is
 // ordered by the library's single stream)

@@ -142,3 +142,52 @@ def workload_inputs(builder, name: str, seed: int = 7):
     W1, W2 = mpo_bond(builder, name)
     L, R, x = heff_inputs(W1, W2, site, m, smax, sigma, seed)
     return dict(W1=W1, W2=W2, L=L, R=R, x=x, site=site, m=m)
+
+
+def env_live_model(env_elems, N: int, stride: int, scale: float = 1.0, elem_bytes: int = 8):
+    """Largest number of environment bytes live at any bond of a sweep under
+    the checkpoint policy of the DMRG driver, from the per-bond sizes of the
+    environments a sweep produced (env_elems: 2 (N-1) entries per sweep in
+    sweep order, the first half moving from the left, so entry i of a left
+    half is TL[i+1] and entry i of a right half is TR[N-2-i]), scaled by
+    `scale` (e.g. 4 for twice the bond dimension).
+
+    At bond `site` of a half-sweep moving from the left, live are: the kept
+    right environments of the previous half-sweep (sites that are multiples
+    of `stride`, and the end), the window rebuilt from the nearest kept one
+    (at most stride - 1), and the left environments kept so far in this
+    half-sweep plus the one just produced; mirrored for the other direction.
+    stride = 1 keeps one full set.  Returns (peak bytes, the bond and
+    half-sweep of the peak)."""
+    env = np.asarray(env_elems, dtype=float) * scale * elem_bytes
+    nb = N - 1
+    nh = len(env) // nb
+    size_L = {}   # site -> bytes of TL[site] / TR[site] most recently produced
+    size_R = {}
+    best = (0.0, None)
+    for h in range(nh):
+        e = env[h * nb:(h + 1) * nb]
+        left = h % 2 == 0
+        if left:
+            for i in range(nb):
+                size_L[i + 1] = e[i]
+        else:
+            for i in range(nb):
+                size_R[nb - 1 - i] = e[i]
+    # sizes of every environment (the last value produced at each site)
+    for h in range(nh):
+        left = h % 2 == 0
+        for i in range(nb):
+            site = i if left else nb - i
+            if left:
+                far = [size_R.get(j, 0.0) for j in range(site + 1, N) if j % stride == 0 or j == N - 1]
+                win = [size_R.get(j, 0.0) for j in range(site + 1, min(N, site + stride + 1))]
+                near = [size_L.get(j, 0.0) for j in range(0, site + 2) if j % stride == 0 or j == site + 1]
+            else:
+                far = [size_L.get(j, 0.0) for j in range(0, site) if j % stride == 0 or j == 0]
+                win = [size_L.get(j, 0.0) for j in range(max(0, site - stride), site)]
+                near = [size_R.get(j, 0.0) for j in range(site - 1, N) if j % stride == 0 or j == site - 1]
+            tot = sum(far) + sum(win) + sum(near)
+            if tot > best[0]:
+                best = (tot, (h, site))
+    return best

@@ -121,15 +121,17 @@ def test_ranged_dmrg():
     rc = P.dmrg_range(c, H, 3, 8, 2, [16], [1e-10], [2], 1e-10)
     assert len(rc["bond_E"]) == 2 * 2 * 5
     assert rc["E"] < rc["E0"] - 0.1
-    # only the links inside the range grew: outside it the bond dimensions are
-    # those the canonicalisation alone gives the product state (its link
-    # sector sets, one state each)
+    # only the links inside the range grew.  Right of it (sites 9..) nothing
+    # moved: the bond dimensions are those the canonicalisation alone gives
+    # the product state (its link sector sets, one state each).  Left of it
+    # the final position(0) re-splits links 1-3 at cutoff 0, where a sector
+    # that lost its partners inside the range drops out: never larger.
     bd = c.bond_dims()
     q = P.mps_product([i % 2 for i in range(N)])
     q.position(3)
     q.position(0)
     bq = q.bond_dims()
-    assert np.array_equal(bd[:4], bq[:4]) and np.array_equal(bd[9:], bq[9:]), (bd, bq)
+    assert np.array_equal(bd[9:], bq[9:]) and np.all(bd[:4] <= bq[:4]), (bd, bq)
     assert max(bd[4:9]) > max(bq[4:9]), (bd, bq)
 
 

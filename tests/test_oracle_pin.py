@@ -136,3 +136,106 @@ def test_order_statement_rejects_a_swapped_loop():
     blocks, _ = contraction_order(L, x, [(2, 0)])
     swapped = sorted(blocks.tolist(), key=lambda t: (t[0], t[1]))
     assert blocks.tolist() != swapped
+
+
+def _dense_state(psi):
+    t = psi.tensor(0)
+    for i in range(1, len(psi)):
+        t = t * psi.tensor(i)
+    return t.to_flat().to_dense().ravel()
+
+
+@pytest.mark.parametrize("N,m,k", [(6, 8, 2), (6, 8, 0), (8, 12, 7), (8, 12, 3)])
+def test_position_keeps_the_state_and_makes_it_canonical(N, m, k):
+    """position(k) (qtt.cpp:932-994, through rc()/lc() at cutoff 0) leaves the
+    state unchanged (the dense contraction of all site tensors, up to norm and
+    sign) and makes every site left of k left-orthonormal and every site
+    right of k right-orthonormal."""
+    from ttgpu.workloads import random_mps
+    psi = O.mps_product([i % 2 for i in range(N)])
+    for i, t in enumerate(random_mps(N, m, 2.0, seed=5)):
+        psi.set_tensor(i, O.OTensor.from_flat(t))
+    psi.set_center(-1)
+    before = _dense_state(psi)
+    psi.position(k)
+    assert psi.center() == k
+    after = _dense_state(psi)
+    b, a = before / np.linalg.norm(before), after / np.linalg.norm(after)
+    assert min(np.linalg.norm(a - b), np.linalg.norm(a + b)) < 1e-13
+    for i in range(N):
+        A = psi.tensor(i).to_flat().to_dense()
+        Dl, d, Dr = A.shape
+        if i < k:
+            M = A.reshape(Dl * d, Dr, order="F")
+            G = M.T @ M
+        elif i > k:
+            M = A.reshape(Dl, d * Dr, order="F")
+            G = M @ M.T
+        else:
+            continue
+        assert np.abs(G - np.eye(G.shape[0])).max() < 1e-13, (i, k)
+
+
+def _end_env(i0, w0, sign_flip=True):
+    """The open-boundary end of an environment (dmrg.cpp:60-74): indices
+    (psi link primed and dagged, MPO link, psi link), every legal block one,
+    then every arrow flipped."""
+    from ttgpu.flat import FlatTensor, legal_blocks
+    a = i0.dag()
+    a.level += 1
+    idx = [a, w0.copy(), i0.copy()]
+    bq = legal_blocks(idx)
+    t = FlatTensor(idx, bq, np.zeros(0))
+    t.data = np.ones(int(t.block_sizes().sum()))
+    t.idx = [ix.dag() for ix in t.idx]
+    return t
+
+
+def test_davidson_reaches_the_lowest_eigenvalue_of_the_dense_operator():
+    """tensor_davidsonIT (tensor_davidson.cpp:192-347) on a two-site H_eff
+    built from environments of a real chain (Hermitian): with many restarts it
+    converges to the lowest eigenvalue of the dense operator (numpy eigh),
+    and with the reference's max_restart = 2 it stays between that value and
+    the starting Rayleigh quotient (variational)."""
+    from ttgpu.workloads import random_mps
+    N = 8
+    H = O.mpo_heisenberg_nn(N)
+    psi = O.mps_product([i % 2 for i in range(N)])
+    for i, t in enumerate(random_mps(N, 8, 2.0, seed=3)):
+        psi.set_tensor(i, O.OTensor.from_flat(t))
+    psi.set_center(-1)
+    psi.position(3)
+    A = [psi.tensor(i) for i in range(N)]
+    W = [H.tensor(i) for i in range(N)]
+    TL = O.OTensor.from_flat(_end_env(A[0].to_flat().idx[0], W[0].to_flat().idx[0]))
+    for i in range(3):
+        TL = O.env_step(A[i], TL, W[i])
+    TR = O.OTensor.from_flat(_end_env(A[N - 1].to_flat().idx[2], W[N - 1].to_flat().idx[3]))
+    for i in range(N - 1, 4, -1):
+        TR = O.env_step(A[i], TR, W[i])
+    big = O.OBig(TL, W[3], W[4], TR)
+    x0 = A[3] * A[4]
+    f = x0.to_flat()
+    n = f.data.size
+    cols = []
+    for j in range(n):
+        e = f.data * 0.0
+        e[j] = 1.0
+        u = O.OTensor.from_flat(type(f)(f.idx, f.block_qi, e))
+        y = big.product(u).to_flat()
+        pos = {tuple(q): b for b, q in enumerate(y.block_qi.tolist())}
+        yo, fo = y.block_offsets(), f.block_offsets()
+        col = np.zeros(n)
+        for b, q in enumerate(f.block_qi.tolist()):
+            if tuple(q) in pos:
+                g = pos[tuple(q)]
+                col[fo[b]:fo[b + 1]] = y.data[yo[g]:yo[g + 1]]
+        cols.append(col)
+    Hd = np.array(cols).T
+    assert np.allclose(Hd, Hd.T, atol=1e-12 * np.abs(Hd).max())
+    lam = np.linalg.eigvalsh(0.5 * (Hd + Hd.T))[0]
+    rq = float(f.data @ Hd @ f.data / (f.data @ f.data))
+    e_many, _ = big.davidson(x0.copy(), max_restart=60, tol=1e-12)
+    assert abs(e_many - lam) < 1e-10 * abs(lam)
+    e2, _ = big.davidson(x0.copy(), max_restart=2, tol=1e-12)
+    assert lam - 1e-12 <= e2 <= rq + 1e-12
