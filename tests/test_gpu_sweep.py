@@ -67,3 +67,19 @@ def test_sweep_m2048_fp64_matches_oracle():
     h = len(gT) // 2
     for a, b in ((slice(0, h), "first"), (slice(h, None), "second")):
         assert np.sum(r["trunc_err"][a]) == pytest.approx(np.sum(gT[a]), rel=1e-2), b
+
+
+def test_sweep_m2048_checkpointed_environments_bit_identical():
+    """The same FP64 sweep with environments kept only at every fourth site
+    (rebuilt when a bond reads them, the memory plan of the m = 8192 runs)
+    gives the same bits as keeping one full set."""
+    r1 = run(T.FP64)
+    P.set_env_checkpoint_stride(4)
+    try:
+        r4 = run(T.FP64)
+    finally:
+        P.set_env_checkpoint_stride(1)
+    assert np.array_equal(r4["bond_E"], r1["bond_E"])
+    assert np.array_equal(r4["kept"], r1["kept"])
+    assert r4["E"] == r1["E"]
+    assert r4["pool_live_max"] < r1["pool_live_max"]
