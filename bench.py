@@ -248,6 +248,8 @@ def main():
                     help="most FP64 sweeps of the converged precision comparison (0 skips it)")
     ap.add_argument("--converge-tol", type=float, default=1e-10,
                     help="energy-per-site change between sweeps that counts as converged")
+    ap.add_argument("--precision-check", type=int, default=1,
+                    help="1: after the sweep leg, the N=20 chain comparison of the precision forms")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a dispatch of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -352,7 +354,8 @@ def main():
 
     sweep = sweep_leg(P, T, args) if args.sweep_m > 0 else None
     if sweep is not None:
-        sweep["precision_check"] = precision_check(P, T)
+        if args.precision_check:
+            sweep["precision_check"] = precision_check(P, T)
         if args.converge_sweeps > 0:
             sweep["precision_converged"] = converged_precision(P, T, args)
 

@@ -494,13 +494,16 @@ __device__ inline void jac_rotation(double app, double aqq, double apq, double& 
 // rotation acts on adjacent rows and columns.  Between rounds the indices
 // move by the fixed permutation slot_next (period 63); a round reads S and
 // Q^T from one buffer and writes the rotated values to their next-round
-// positions in the other.  A round:
-//   1. 32 threads: rotation of each slot from its 2x2 diagonal block;
+// positions in the other.  A round, one barrier:
+//   1. every wave: lane k < 32 computes the rotation of slot k from its 2x2
+//      diagonal block (the four waves compute the same values from the same
+//      LDS words), and each thread takes the rotations it needs from those
+//      lanes by cross-lane shuffles, so no rotation table passes through LDS;
 //   2. all threads: S <- M S M^T (thread t: rows 2k, 2k+1, k = t/8, and the
 //      column pairs k' = t%8 + 8u) and Q^T <- M Q^T (the same rows).
 constexpr int kSP = 72;                    // LDS row pitch (doubles) of S and Q^T
 constexpr int kSlotMat = kSub * kSP;
-constexpr size_t kSolveLds = (4 * kSlotMat + 2 * kBJ) * sizeof(double) + (kSub + 1) * sizeof(int);
+constexpr size_t kSolveLds = 4 * kSlotMat * sizeof(double) + (kSub + 1) * sizeof(int);
 __device__ inline int slot_of(int x) { return x < kBJ ? 2 * x : 2 * (kSub - 1 - x) + 1; }
 __device__ inline int slot_next(int p) {
   const int k = p >> 1;
@@ -514,8 +517,7 @@ __global__ void __launch_bounds__(kSolveThreads)
 k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const int* active, int* flag,
              int* rotated, const double* Gp, double* Qp) {
   extern __shared__ __attribute__((aligned(16))) double bj_lds[];
-  double* cs = bj_lds + 4 * kSlotMat;          // (c_k, s_k) at 2k, 2k+1
-  int* perm = (int*)(cs + 2 * kBJ);
+  int* perm = (int*)(bj_lds + 4 * kSlotMat);
   int* sig = perm + kSub;
   const int2 it = items[blockIdx.x];
   const int li = it.x, k = it.y;
@@ -558,7 +560,7 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
     rotated[blockIdx.x] = 1;
   }
   typedef double d2 __attribute__((ext_vector_type(2)));
-  const int kk = tid >> 3, c8 = tid & 7;
+  const int kk = tid >> 3, c8 = tid & 7, lane = tid & 63;
   const int r0 = 2 * kk, r1 = 2 * kk + 1;
   const int nr0 = slot_next(r0) * kSP, nr1 = slot_next(r1) * kSP;
   int ncol[4][2];
@@ -572,13 +574,6 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
     double* Sn = bj_lds + ((round & 1) ^ 1) * kSlotMat;
     const double* Qc = bj_lds + (2 + (round & 1)) * kSlotMat;
     double* Qn = bj_lds + (2 + ((round & 1) ^ 1)) * kSlotMat;
-    if (tid < kBJ) {
-      const int p = 2 * tid;
-      double c, s;
-      jac_rotation(Sc[p * kSP + p], Sc[(p + 1) * kSP + p + 1], Sc[p * kSP + p + 1], c, s);
-      *(d2*)&cs[p] = d2{c, s};
-    }
-    __syncthreads();
     d2 x0[4], x1[4], q0[4], q1[4], cp[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -586,10 +581,17 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
       x1[u] = *(const d2*)&Sc[r1 * kSP + 2 * c8 + 16 * u];
       q0[u] = *(const d2*)&Qc[r0 * kSP + 2 * c8 + 16 * u];
       q1[u] = *(const d2*)&Qc[r1 * kSP + 2 * c8 + 16 * u];
-      cp[u] = *(const d2*)&cs[2 * c8 + 16 * u];
     }
-    const d2 ck = *(const d2*)&cs[2 * kk];
-    const double c1 = ck.x, s1 = ck.y;
+    // lane k (k < 32) of every wave: the rotation of slot k; lanes 32..63
+    // compute slot k - 32 again (same words, no divergence)
+    double cl, sl;
+    {
+      const int p = 2 * (lane & (kBJ - 1));
+      jac_rotation(Sc[p * kSP + p], Sc[(p + 1) * kSP + p + 1], Sc[p * kSP + p + 1], cl, sl);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) cp[u] = d2{__shfl(cl, c8 + 8 * u), __shfl(sl, c8 + 8 * u)};
+    const double c1 = __shfl(cl, kk), s1 = __shfl(sl, kk);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       // rows 2k, 2k+1 of the 2x2 block (k, k' = c8 + 8u), then its columns
