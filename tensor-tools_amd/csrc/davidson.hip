@@ -190,9 +190,9 @@ int ttg_dot_matched(ttg_tensor x, ttg_tensor y, double* out) {
     hipLaunchKernelGGL(k_dot_mapped, dim3(grid), dim3(256), 0, s, nx, map, a.d_off, b.d_off,
                        x->d_data, y->d_data, part);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(512), 0, s, part, grid, part + grid);
-    TTG_HIP(hipMemcpyAsync(&r, part + grid, sizeof(double), hipMemcpyDeviceToHost, s));
+    const D2HPiece pc{&r, part + grid, sizeof(double)};
+    TTG_TRY(d2h_wait(&pc, 1));
   }
-  TTG_HIP(hipStreamSynchronize(s));
   *out = r;
   return 0;
 }
@@ -204,12 +204,10 @@ int ttg_small_eigh(int n, const double* M, double* evals, double* evecs) {
   Scratch sc;
   double* d = sc.get<double>(n * n * 2 + n);
   TTG_TRY(sc.ok("small_eigh"));
-  TTG_HIP(hipMemcpyAsync(d, M, sizeof(double) * n * n, hipMemcpyHostToDevice, stream()));
+  TTG_TRY(h2d(d, M, sizeof(double) * n * n));
   hipLaunchKernelGGL(k_small_eigh, dim3(1), dim3(64), 0, stream(), n, d, d + 2 * n * n, d + n * n);
-  TTG_HIP(hipMemcpyAsync(evecs, d + n * n, sizeof(double) * n * n, hipMemcpyDeviceToHost, stream()));
-  TTG_HIP(hipMemcpyAsync(evals, d + 2 * n * n, sizeof(double) * n, hipMemcpyDeviceToHost, stream()));
-  TTG_HIP(hipStreamSynchronize(stream()));
-  return 0;
+  const D2HPiece pc[2] = {{evecs, d + n * n, sizeof(double) * n * n}, {evals, d + 2 * n * n, sizeof(double) * n}};
+  return d2h_wait(pc, 2);
 }
 
 int ttg_embed(ttg_tensor src, ttg_tensor dst) {
@@ -239,8 +237,10 @@ int ttg_embed(ttg_tensor src, ttg_tensor dst) {
     hipLaunchKernelGGL(k_embed_copy, dim3(std::min(ns, 2048)), dim3(256), 0, s, ns, map,
                        src->m->d_off, dst->m->d_off, src->d_data, dst->d_data);
   }
-  TTG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  {
+    const D2HPiece pc{&herr, err, sizeof(int)};
+    TTG_TRY(d2h_wait(&pc, 1));
+  }
   mark_written(dst);
   TTG_REQUIRE(herr == 0, "embed: source block missing from destination block list");
   return 0;

@@ -96,8 +96,10 @@ static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
   if (nc)
     hipLaunchKernelGGL(k_remap, dim3((nc + 255) / 256), dim3(256), 0, s, p3->C->view(), skeys, sids,
                        nx, xm->d_off, (const int*)nullptr, p3->d_cblk, err);
-  TTG_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  {
+    const D2HPiece pc{&herr, err, sizeof(int)};
+    TTG_TRY(d2h_wait(&pc, 1));
+  }
   TTG_REQUIRE(herr == 0, "heff: the product has blocks outside x's block list "
                          "(give x the full legal block list, ttg_tensor_init_blocks)");
   return 0;

@@ -266,14 +266,36 @@ __device__ inline int foldk(int n, const int* ext, const int64_t* sa, const int6
   return ncomp;
 }
 
+// One wave per result block.  The block's tasks are its mid tuples in
+// ascending order (qtensor.cpp:696-722); the lanes take 64 mids at a time,
+// and each task's position in the list and the k-tile offset of its walk
+// are exclusive prefix sums over the lanes in mid order, so the task list is
+// the one a walk over the mids in order gives.
+__device__ inline int64_t wave_sum_i64(int64_t v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+__device__ inline double wave_sum_f64(double v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+__device__ inline int64_t wave_max_i64(int64_t v) {
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int64_t o = __shfl_xor(v, d);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, PairSpec ps,
                                                      KeyArrays k, const uint64_t* uL,
                                                      const uint64_t* uR, const uint64_t* uM,
                                                      QTab q, int nQ, int nblocksC, int phase,
                                                      BlkArrays ba, EmitArrays ea, PlanHdr* hdr,
                                                      int nA, int nB, int TK) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nblocksC) return;
+  const int lane = threadIdx.x & 63;
+  const int b = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+  if (b >= nblocksC) return;   // the whole wave
   // q group containing b: last t with qbase[t] <= b
   int lo = 0, hi = nQ;
   while (lo < hi) {
@@ -299,102 +321,127 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
   int64_t M = 1, N = 1;
   for (int j = 0; j < ps.nfa; j++) M *= A.idx[3 * (A.idx_off[ps.fa[j]] + lq[j]) + 2];
   for (int j = 0; j < ps.nfb; j++) N *= B.idx[3 * (B.idx_off[ps.fb[j]] + rq[j]) + 2];
-  if (phase == 0) {
+  if (phase == 0 && lane == 0) {
     for (int j = 0; j < ps.nfa; j++) ba.c_qi[(size_t)b * rankC + j] = lq[j];
     for (int j = 0; j < ps.nfb; j++) ba.c_qi[(size_t)b * rankC + ps.nfa + j] = rq[j];
     ba.size[b] = M * N;
   }
-  int64_t nt = 0;
+  const int64_t tw = phase == 1 ? ba.task_off[b] : 0;
+  int64_t nt = 0;   // tasks of the block so far (uniform)
+  int64_t kt = 0;   // k-tiles of the block's walk so far (uniform)
+  // per-lane partial sums and maxima, reduced over the wave at the end
   double fl = 0.0;
-  int64_t tw = phase == 1 ? ba.task_off[b] : 0;
   int64_t sumK = 0;
-  int64_t kt = 0;  // running k-tile count of the block's walk
-  int votes[4] = {0, 0, 0, 0};
+  int64_t votes[4] = {0, 0, 0, 0};
   int64_t ktmax = 0, opmax = 0;
-  int bad = 0;
-  for (int m = q.qM0[t]; m < q.qM1[t]; m++) {
-    const uint64_t mid = uM[m] & 0xffffffffull;
-    const int ia = find_sorted(k.sSA, nA, (left << 32) | mid);
-    if (ia < 0) continue;
-    const int ib = find_sorted(k.sSB, nB, (mid << 32) | right);
-    if (ib < 0) continue;
-    const int ablk = k.svSA[ia], bblk = k.svSB[ib];
-    // strides of the stored (unpermuted) blocks
-    int dA[MAXR], dB[MAXR];
-    int64_t sA[MAXR], sB[MAXR];
-    int64_t acc = 1;
-    for (int j = 0; j < A.rank; j++) {
-      dA[j] = A.idx[3 * (A.idx_off[j] + A.qi[(size_t)ablk * A.rank + j]) + 2];
-      sA[j] = acc;
-      acc *= dA[j];
-    }
-    acc = 1;
-    for (int j = 0; j < B.rank; j++) {
-      dB[j] = B.idx[3 * (B.idx_off[j] + B.qi[(size_t)bblk * B.rank + j]) + 2];
-      sB[j] = acc;
-      acc *= dB[j];
-    }
-    int em[MAXR], ek[MAXR], en[MAXR];
-    int64_t smA[MAXR], skA[MAXR], skB[MAXR], snB[MAXR];
-    int64_t K = 1;
-    for (int j = 0; j < ps.nfa; j++) { em[j] = dA[ps.fa[j]]; smA[j] = sA[ps.fa[j]]; }
-    for (int p = 0; p < ps.np; p++) {
-      ek[p] = dB[ps.pb[p]];
-      skA[p] = sA[ps.pa[p]];
-      skB[p] = sB[ps.pb[p]];
-      K *= ek[p];
-    }
-    for (int j = 0; j < ps.nfb; j++) { en[j] = dB[ps.fb[j]]; snB[j] = sB[ps.fb[j]]; }
-    GemmTask g;
-    g.a_off = A.off[ablk];
-    g.b_off = B.off[bblk];
-    g.K = (int)K;
-    int cm = fold2(ps.nfa, em, smA, g.em0, g.sam0, g.sam1);
-    int cn = fold2(ps.nfb, en, snB, g.en0, g.sbn0, g.sbn1);
-    int ck = foldk(ps.np, ek, skA, skB, g.ek0, g.sak0, g.sak1, g.sbk0, g.sbk1);
-    {  // reversed contraction order: taken when it folds better, or when it
-       // puts the longer component first (the GEMM walks k in chunks of the
-       // first component, so a short first component wastes chunk slots)
-      int ekr[MAXR];
-      int64_t skAr[MAXR], skBr[MAXR];
-      for (int p = 0; p < ps.np; p++) {
-        ekr[p] = ek[ps.np - 1 - p];
-        skAr[p] = skA[ps.np - 1 - p];
-        skBr[p] = skB[ps.np - 1 - p];
+  int64_t bad = 0;
+  for (int m0 = q.qM0[t]; m0 < q.qM1[t]; m0 += 64) {
+    const int m = m0 + lane;
+    bool valid = false;
+    GemmTask g{};
+    int64_t K = 1, ktl = 0;
+    int ablk = 0, bblk = 0;
+    if (m < q.qM1[t]) {
+      const uint64_t mid = uM[m] & 0xffffffffull;
+      const int ia = find_sorted(k.sSA, nA, (left << 32) | mid);
+      const int ib = ia < 0 ? -1 : find_sorted(k.sSB, nB, (mid << 32) | right);
+      if (ib >= 0) {
+        valid = true;
+        ablk = k.svSA[ia];
+        bblk = k.svSB[ib];
+        // strides of the stored (unpermuted) blocks
+        int dA[MAXR], dB[MAXR];
+        int64_t sA[MAXR], sB[MAXR];
+        int64_t acc = 1;
+        for (int j = 0; j < A.rank; j++) {
+          dA[j] = A.idx[3 * (A.idx_off[j] + A.qi[(size_t)ablk * A.rank + j]) + 2];
+          sA[j] = acc;
+          acc *= dA[j];
+        }
+        acc = 1;
+        for (int j = 0; j < B.rank; j++) {
+          dB[j] = B.idx[3 * (B.idx_off[j] + B.qi[(size_t)bblk * B.rank + j]) + 2];
+          sB[j] = acc;
+          acc *= dB[j];
+        }
+        int em[MAXR], ek[MAXR], en[MAXR];
+        int64_t smA[MAXR], skA[MAXR], skB[MAXR], snB[MAXR];
+        for (int j = 0; j < ps.nfa; j++) { em[j] = dA[ps.fa[j]]; smA[j] = sA[ps.fa[j]]; }
+        for (int p = 0; p < ps.np; p++) {
+          ek[p] = dB[ps.pb[p]];
+          skA[p] = sA[ps.pa[p]];
+          skB[p] = sB[ps.pb[p]];
+          K *= ek[p];
+        }
+        for (int j = 0; j < ps.nfb; j++) { en[j] = dB[ps.fb[j]]; snB[j] = sB[ps.fb[j]]; }
+        g.a_off = A.off[ablk];
+        g.b_off = B.off[bblk];
+        g.K = (int)K;
+        int cm = fold2(ps.nfa, em, smA, g.em0, g.sam0, g.sam1);
+        int cn = fold2(ps.nfb, en, snB, g.en0, g.sbn0, g.sbn1);
+        int ck = foldk(ps.np, ek, skA, skB, g.ek0, g.sak0, g.sak1, g.sbk0, g.sbk1);
+        {  // reversed contraction order: taken when it folds better, or when it
+           // puts the longer component first (the GEMM walks k in chunks of the
+           // first component, so a short first component wastes chunk slots)
+          int ekr[MAXR];
+          int64_t skAr[MAXR], skBr[MAXR];
+          for (int p = 0; p < ps.np; p++) {
+            ekr[p] = ek[ps.np - 1 - p];
+            skAr[p] = skA[ps.np - 1 - p];
+            skBr[p] = skB[ps.np - 1 - p];
+          }
+          int e0r;
+          int64_t a0r, a1r, b0r, b1r;
+          const int ckr = foldk(ps.np, ekr, skAr, skBr, e0r, a0r, a1r, b0r, b1r);
+          if ((ck > 2 && ckr <= 2) || (ck == 2 && ckr == 2 && e0r > g.ek0)) {
+            ck = ckr;
+            g.ek0 = e0r; g.sak0 = a0r; g.sak1 = a1r; g.sbk0 = b0r; g.sbk1 = b1r;
+          }
+        }
+        if (cm > 2 || cn > 2 || ck > 2) bad = 1;
+        ktl = (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK);
+        if (phase == 0) {
+          votes[0] += (g.sam0 == 1) || M == 1;
+          votes[1] += (g.sak0 == 1) || K == 1;
+          votes[2] += (g.sbk0 == 1) || K == 1;
+          votes[3] += (g.sbn0 == 1) || N == 1;
+        }
+        ktmax = K > ktmax ? K : ktmax;
+        opmax = M * K > opmax ? M * K : opmax;
+        opmax = K * N > opmax ? K * N : opmax;
+        // K walk in TK-wide chunks of the first k component (gemm.hip)
+        sumK += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK * TK);
+        fl += 2.0 * (double)M * (double)N * (double)K;
       }
-      int e0r;
-      int64_t a0r, a1r, b0r, b1r;
-      const int ckr = foldk(ps.np, ekr, skAr, skBr, e0r, a0r, a1r, b0r, b1r);
-      if ((ck > 2 && ckr <= 2) || (ck == 2 && ckr == 2 && e0r > g.ek0)) {
-        ck = ckr;
-        g.ek0 = e0r; g.sak0 = a0r; g.sak1 = a1r; g.sbk0 = b0r; g.sbk1 = b1r;
-      }
     }
-    if (cm > 2 || cn > 2 || ck > 2) bad = 1;
-    if (phase == 0) {
-      votes[0] += (g.sam0 == 1) || M == 1;
-      votes[1] += (g.sak0 == 1) || K == 1;
-      votes[2] += (g.sbk0 == 1) || K == 1;
-      votes[3] += (g.sbn0 == 1) || N == 1;
-    } else {
-      g.kt0 = (int)kt;
+    // position and k-tile offset: exclusive prefix sums over the lanes
+    const uint64_t vm = __ballot(valid);
+    const int pos = __popcll(vm & ((1ull << lane) - 1ull));
+    int64_t incl = valid ? ktl : 0;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int64_t y = __shfl_up(incl, d);
+      if (lane >= d) incl += y;
+    }
+    if (valid && phase == 1) {
+      g.kt0 = (int)(kt + incl - ktl);
       g.pad = 0;
-      ea.tasks[tw + nt] = g;
+      ea.tasks[tw + nt + pos] = g;
       if (ea.tuples) {
-        int* tu = ea.tuples + 6 * (tw + nt);
+        int* tu = ea.tuples + 6 * (tw + nt + pos);
         tu[0] = b; tu[1] = ablk; tu[2] = bblk; tu[3] = (int)M; tu[4] = (int)N; tu[5] = (int)K;
       }
     }
-    nt++;
-    ktmax = K > ktmax ? K : ktmax;
-    opmax = M * K > opmax ? M * K : opmax;
-    opmax = K * N > opmax ? K * N : opmax;
-    kt += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK);
-    // K walk in TK-wide chunks of the first k component (gemm.hip)
-    sumK += (int64_t)(K / g.ek0) * (int64_t)((g.ek0 + TK - 1) / TK * TK);
-    fl += 2.0 * (double)M * (double)N * (double)K;
+    nt += __popcll(vm);
+    kt += __shfl(incl, 63);
   }
   if (phase == 0) {
+    fl = wave_sum_f64(fl);
+    sumK = wave_sum_i64(sumK);
+    for (int v = 0; v < 4; v++) votes[v] = wave_sum_i64(votes[v]);
+    ktmax = wave_max_i64(ktmax);
+    opmax = wave_max_i64(opmax);
+    bad = wave_max_i64(bad);
+    if (lane != 0) return;
     ba.ntask[b] = nt;
     ba.flops[b] = fl;
     for (int c = 0; c < TILE_NCFG; c++) {
@@ -412,7 +459,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
     for (int v = 0; v < 4; v++)
       if (votes[v]) atomicAdd((unsigned long long*)&hdr->votes[v], (unsigned long long)votes[v]);
     if (bad) atomicAdd((unsigned long long*)&hdr->err, 1ull);
-  } else {
+  } else if (lane == 0) {
     CBlock cb;
     cb.c_off = ba.c_off[b];
     cb.M = (int)M;
@@ -571,7 +618,6 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
   // so consecutive items (one XCD's run, see xcd_first_item) reuse its L2.
   // A longest-walk-first order balanced the tail no better and ran the
   // m=2048 and m=8192 operators 1-4% slower.
-  hipStream_t s = stream();
   p.ntiles = (int)tiles.size();
   p.ngroups = (int)groups.size();
   p.nslots = nslots;
@@ -580,13 +626,8 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
   const size_t acc = p.prec == TTG_FP64 ? sizeof(double) : sizeof(float);
   p.d_part = nslots ? dmalloc((size_t)nslots * TM * TN * acc) : nullptr;
   TTG_REQUIRE(p.d_tiles && p.d_groups && (p.d_part || !nslots), "plan: work list allocation failed");
-  if (!tiles.empty())
-    TTG_HIP(hipMemcpyAsync(p.d_tiles, tiles.data(), sizeof(Tile) * tiles.size(),
-                           hipMemcpyHostToDevice, s));
-  if (!groups.empty())
-    TTG_HIP(hipMemcpyAsync(p.d_groups, groups.data(), sizeof(SplitGroup) * groups.size(),
-                           hipMemcpyHostToDevice, s));
-  TTG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
+  TTG_TRY(h2d_vec(p.d_tiles, tiles));
+  TTG_TRY(h2d_vec(p.d_groups, groups));
   return 0;
 }
 
@@ -594,10 +635,8 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
 int rebuild_tiles(ttg_plan_s& p, int cfg) {
   TTG_REQUIRE(cfg >= 0 && cfg < TILE_NCFG, "plan: bad tile shape");
   std::vector<CBlock> hcb(p.nblocksC);
-  hipStream_t s = stream();
-  if (p.nblocksC)
-    TTG_HIP(hipMemcpyAsync(hcb.data(), p.d_cblk, sizeof(CBlock) * p.nblocksC, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  const D2HPiece pc{hcb.data(), p.d_cblk, sizeof(CBlock) * p.nblocksC};
+  TTG_TRY(d2h_wait(&pc, 1));
   dfree(p.d_tiles); dfree(p.d_groups); dfree(p.d_part);
   p.d_tiles = nullptr; p.d_groups = nullptr; p.d_part = nullptr;
   p.ngroups = 0;
@@ -705,7 +744,7 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   TTG_TRY(sc.ok("plan"));
   ba.c_qi = C->d_qi;
   EmitArrays ea{};
-  const unsigned gb = (unsigned)std::max<int64_t>(1, (nblocksC + 255) / 256);
+  const unsigned gb = (unsigned)std::max<int64_t>(1, (nblocksC + 3) / 4);   // one wave per result block
   hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
                      uM, q, nQ, (int)nblocksC, 0, ba, ea, d_hdr, nA, nB, tk_of(prec));
   hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, (int)nblocksC, ba.size, ba.ntask,
@@ -753,9 +792,8 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
                      uM, q, nQ, (int)nblocksC, 1, ba, ea, d_hdr, nA, nB, tk_of(prec));
   std::vector<CBlock> hcb(nblocksC);
-  if (nblocksC)
-    TTG_HIP(hipMemcpyAsync(hcb.data(), p->d_cblk, sizeof(CBlock) * nblocksC, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  const D2HPiece pc{hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)nblocksC};
+  TTG_TRY(d2h_wait(&pc, 1));
   TTG_TRY(build_tiles(*p, hcb));
   *out = p.release();
   return 0;

@@ -1,9 +1,10 @@
-// runtime.hip -- device binding, stream, error string, caching pool and the
-// host-side phase timers.
+// runtime.hip -- device binding, stream, error string, caching pool, pinned
+// staging of small host-device copies and the host-side phase timers.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 
@@ -133,6 +134,75 @@ int side_join() {
   if (!g_rt.side) return 0;
   TTG_HIP(hipEventRecord(g_rt.join, g_rt.side));
   TTG_HIP(hipStreamWaitEvent(g_rt.stream, g_rt.join, 0));
+  return 0;
+}
+
+// ---------------------------------------------------------------- staging
+// Small host-to-device copies go through a pinned ring: the bytes are copied
+// into the ring and the device copy is queued from there, so the call returns
+// at once and the caller's buffer may go out of scope (a copy queued from
+// pageable memory makes the host wait for the stream).  Every ring copy is
+// queued on the library stream; a copy that would wrap the ring first waits
+// for that stream, which has then finished every earlier copy out of the
+// ring.  Copies above a quarter of the ring are queued directly.
+namespace {
+constexpr size_t kRingBytes = size_t(8) << 20;
+struct Ring {
+  char* p = nullptr;
+  size_t off = 0;
+} g_ring;
+struct Readback {
+  char* p = nullptr;
+  size_t cap = 0;
+} g_rb;
+}  // namespace
+
+int h2d(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return 0;
+  TTG_TRY(ensure_init());
+  hipStream_t s = g_rt.stream;
+  if (bytes > kRingBytes / 4) {
+    TTG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipStreamSynchronize(s));   // the caller's buffer may be released on return
+    return 0;
+  }
+  if (!g_ring.p) TTG_HIP(hipHostMalloc((void**)&g_ring.p, kRingBytes));
+  size_t off = (g_ring.off + 255) & ~size_t(255);
+  if (off + bytes > kRingBytes) {
+    TTG_HIP(hipStreamSynchronize(s));
+    off = 0;
+  }
+  memcpy(g_ring.p + off, src, bytes);
+  TTG_HIP(hipMemcpyAsync(dst, g_ring.p + off, bytes, hipMemcpyHostToDevice, s));
+  g_ring.off = off + bytes;
+  return 0;
+}
+
+int d2h_wait(const D2HPiece* pieces, int n) {
+  TTG_TRY(ensure_init());
+  hipStream_t s = g_rt.stream;
+  size_t total = 0;
+  for (int i = 0; i < n; i++) total += (pieces[i].bytes + 255) & ~size_t(255);
+  if (total > g_rb.cap) {
+    if (g_rb.p) TTG_HIP(hipHostFree(g_rb.p));
+    g_rb.p = nullptr;
+    g_rb.cap = 0;
+    const size_t cap = std::max<size_t>(total, size_t(1) << 20);
+    TTG_HIP(hipHostMalloc((void**)&g_rb.p, cap));
+    g_rb.cap = cap;
+  }
+  size_t off = 0;
+  for (int i = 0; i < n; i++) {
+    if (pieces[i].bytes)
+      TTG_HIP(hipMemcpyAsync(g_rb.p + off, pieces[i].src, pieces[i].bytes, hipMemcpyDeviceToHost, s));
+    off += (pieces[i].bytes + 255) & ~size_t(255);
+  }
+  TTG_HIP(hipStreamSynchronize(s));
+  off = 0;
+  for (int i = 0; i < n; i++) {
+    if (pieces[i].bytes) memcpy(pieces[i].dst, g_rb.p + off, pieces[i].bytes);
+    off += (pieces[i].bytes + 255) & ~size_t(255);
+  }
   return 0;
 }
 

@@ -795,10 +795,10 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   int* rot = sc.get<int>(pitems.size());
   TTG_TRY(sc.ok("svd: block Jacobi"));
   std::vector<int> hact(ns, 1);
-  TTG_HIP(hipMemcpyAsync(d_sec, sec.data(), sizeof(BJSector) * ns, hipMemcpyHostToDevice, s));
-  TTG_HIP(hipMemcpyAsync(d_pit, pitems.data(), sizeof(int2) * pitems.size(), hipMemcpyHostToDevice, s));
-  TTG_HIP(hipMemcpyAsync(d_tit, titems.data(), sizeof(int4) * titems.size(), hipMemcpyHostToDevice, s));
-  TTG_HIP(hipMemcpyAsync(act, hact.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
+  TTG_TRY(h2d_vec(d_sec, sec));
+  TTG_TRY(h2d_vec(d_pit, pitems));
+  TTG_TRY(h2d_vec(d_tit, titems));
+  TTG_TRY(h2d_vec(act, hact));
   TTG_HIP(hipFuncSetAttribute((const void*)k_bjac_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kSolveLds));
   // largest diagonal per sector and the convergence floor, on the device
@@ -814,17 +814,17 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
                          (int)titems.size(), r, act, rot, Gp, Vp, Qp);
     }
     std::vector<int> hf(ns);
-    TTG_HIP(hipMemcpyAsync(hf.data(), flag, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
-    TTG_HIP(hipStreamSynchronize(s));
+    const D2HPiece pc{hf.data(), flag, sizeof(int) * ns};
+    TTG_TRY(d2h_wait(&pc, 1));
     int any = 0;
     for (int li = 0; li < ns; li++) any |= hf[li];
     if (!any) break;
-    TTG_HIP(hipMemcpyAsync(act, hf.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
+    TTG_TRY(h2d_vec(act, hf));
   }
   hipLaunchKernelGGL(k_bjac_finish, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, d_eoff, Gp, Vp,
                      Vev, ev);
-  TTG_HIP(hipStreamSynchronize(s));
-  return 0;
+  TTG_HIP(hipGetLastError());
+  return 0;   // the scratch goes back to the pool in stream order
 }
 
 // 4c. LR preconditioning of the large sectors (cutoff > 0).  A sector's
@@ -1336,14 +1336,11 @@ static int run_host_gemm(HostGemm& hg, const double* A, const double* B, double*
   p.d_cblk = dalloc<CBlock>(hg.cblk.size());
   TTG_REQUIRE(p.d_tasks && p.d_cblk, (std::string("svd: ") + ttg_last_error()).c_str());
   hipStream_t s = stream();
-  TTG_HIP(hipMemcpyAsync(p.d_tasks, hg.tasks.data(), sizeof(GemmTask) * hg.tasks.size(),
-                         hipMemcpyHostToDevice, s));
-  TTG_HIP(hipMemcpyAsync(p.d_cblk, hg.cblk.data(), sizeof(CBlock) * hg.cblk.size(),
-                         hipMemcpyHostToDevice, s));
+  TTG_TRY(h2d_vec(p.d_tasks, hg.tasks));
+  TTG_TRY(h2d_vec(p.d_cblk, hg.cblk));
   TTG_TRY(build_tiles(p, hg.cblk));
   TTG_TRY(gemm_launch(p, A, B, C, true, s));
-  TTG_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
-  return 0;   // p's destructor releases the device arrays
+  return 0;   // p's destructor returns the device arrays to the pool (stream-ordered reuse)
 }
 
 static GemmTask gtask(int64_t a_off, int64_t b_off, int em0, int64_t sam0, int ek0, int64_t sak0,
@@ -1361,9 +1358,7 @@ static GemmTask gtask(int64_t a_off, int64_t b_off, int em0, int64_t sam0, int e
 template <class T>
 static T* up(Scratch& sc, const std::vector<T>& v) {
   T* d = sc.get<T>(std::max<size_t>(v.size(), 1));
-  if (d && !v.empty() &&
-      hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream()) != hipSuccess) {
-    set_error("host-to-device copy failed");
+  if (d && !v.empty() && h2d_vec(d, v) != 0) {
     sc.hold(nullptr);   // marks the group failed
     return nullptr;
   }
@@ -1397,7 +1392,7 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
   int* rank = sc.get<int>(ns);
   double* dmax = sc.get<double>(ns);
   TTG_TRY(sc.ok("svd: LR step"));
-  TTG_HIP(hipMemcpyAsync(d_ps, ps.data(), sizeof(PcSector) * ns, hipMemcpyHostToDevice, s));
+  TTG_TRY(h2d_vec(d_ps, ps));
   hipLaunchKernelGGL(k_pchol_init, dim3(ns), dim3(kPcThreads), 0, s, d_ps, G, dg, pg, rank, dmax);
   TTG_HIP(hipFuncSetAttribute((const void*)k_pchol_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
@@ -1408,8 +1403,10 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
     hipLaunchKernelGGL(k_pchol_update, dim3(nt * nt, ns), dim3(256), 0, s, d_ps, k0, nb, rank, G, R);
   }
   std::vector<int> rk(ns);
-  TTG_HIP(hipMemcpyAsync(rk.data(), rank, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  {
+    const D2HPiece pc{rk.data(), rank, sizeof(int) * ns};
+    TTG_TRY(d2h_wait(&pc, 1));
+  }
   std::vector<int> r2(ns);
   std::vector<int64_t> goff2(ns + 1, 0), poff2(ns + 1, 0), eoff2(ns + 1, 0);
   int64_t lds_need = 1;
@@ -1469,7 +1466,7 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
     TTG_TRY(run_host_gemm(hv, R, W2, Vev));
   }
   hipLaunchKernelGGL(k_lr_finish, dim3(nmax, ns), dim3(256), 0, s, d_ps, d_r2, d_eoff2, ev2, d_eoff, Vev, ev);
-  TTG_HIP(hipStreamSynchronize(s));
+  TTG_HIP(hipGetLastError());
   return 0;
 }
 
@@ -1511,20 +1508,24 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   hipLaunchKernelGGL(k_sort_keys, dim3(g1), dim3(256), 0, s, kr, ids, nb, sr, sid);
   hipLaunchKernelGGL(k_svd_unique, dim3(1), dim3(1024), 0, s, m.view(), nleft, sl, sr, nb, t);
   int64_t hdr[4] = {0, 0, 0, 0};
-  TTG_HIP(hipMemcpyAsync(hdr, t.hdr, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  {
+    const D2HPiece pc{hdr, t.hdr, sizeof(int64_t) * 3};
+    TTG_TRY(d2h_wait(&pc, 1));
+  }
   const int nuL = (int)hdr[0], nuR = (int)hdr[1], nQ = (int)hdr[2];
   std::vector<int64_t> rows(nQ), cols(nQ);
   std::vector<uint32_t> qcs(nQ);
   std::vector<int> qL0(nQ), qL1(nQ), qR0(nQ), qR1(nQ);
-  TTG_HIP(hipMemcpyAsync(rows.data(), t.rows, sizeof(int64_t) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipMemcpyAsync(cols.data(), t.cols, sizeof(int64_t) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipMemcpyAsync(qcs.data(), t.qc, sizeof(uint32_t) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipMemcpyAsync(qL0.data(), t.qL0, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipMemcpyAsync(qL1.data(), t.qL1, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipMemcpyAsync(qR0.data(), t.qR0, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipMemcpyAsync(qR1.data(), t.qR1, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  {
+    const D2HPiece pc[7] = {{rows.data(), t.rows, sizeof(int64_t) * nQ},
+                            {cols.data(), t.cols, sizeof(int64_t) * nQ},
+                            {qcs.data(), t.qc, sizeof(uint32_t) * nQ},
+                            {qL0.data(), t.qL0, sizeof(int) * nQ},
+                            {qL1.data(), t.qL1, sizeof(int) * nQ},
+                            {qR0.data(), t.qR0, sizeof(int) * nQ},
+                            {qR1.data(), t.qR1, sizeof(int) * nQ}};
+    TTG_TRY(d2h_wait(pc, 7));
+  }
   // ---- host layout of the per-sector work arrays
   const bool fromLeft = direction == TTG_MOVE_FROM_LEFT;
   std::vector<int64_t> moff(nQ + 1, 0), goff(nQ + 1, 0), poff(nQ + 1, 0), eoff(nQ + 1, 0);
@@ -1592,10 +1593,10 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     int* bad = sc.get<int>(1);
     TTG_TRY(sc.ok("svd"));
     int hbad = 0;
-    TTG_HIP(hipMemcpyAsync(bad, &hbad, sizeof(int), hipMemcpyHostToDevice, s));
+    TTG_TRY(h2d(bad, &hbad, sizeof(int)));
     hipLaunchKernelGGL(k_svd_finite, dim3(nQ), dim3(256), 0, s, nQ, d_nq, d_goff, G, bad);
-    TTG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-    TTG_HIP(hipStreamSynchronize(s));
+    const D2HPiece pc{&hbad, bad, sizeof(int)};
+    TTG_TRY(d2h_wait(&pc, 1));
     TTG_REQUIRE(!hbad, "svd: a sector's Gram matrix is not finite (tensor entries above ~1e154, or NaN)");
   }
   // ---- 4. eigensolver
@@ -1659,9 +1660,10 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   }
   std::vector<int> nd(nQ);
   double th[2];
-  TTG_HIP(hipMemcpyAsync(nd.data(), newdim, sizeof(int) * nQ, hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipMemcpyAsync(th, tout, sizeof(th), hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  {
+    const D2HPiece pc[2] = {{nd.data(), newdim, sizeof(int) * nQ}, {th, tout, sizeof(th)}};
+    TTG_TRY(d2h_wait(pc, 2));
+  }
   if (trunc_err) *trunc_err = th[0];
   if (kept) *kept = (int64_t)th[1];
   // ---- 6. outputs: index tables (left..., mid) (mid, right...) (mid)
@@ -1714,9 +1716,8 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     std::vector<int64_t> soffv(nm + 1, 0);
     for (int q = 0, k = 0; q < nQ; q++)
       if (nd[q] > 0) { soffv[k + 1] = soffv[k] + nd[q]; k++; }
-    TTG_HIP(hipMemcpyAsync(mid->d_qi, sqi.data(), sizeof(int) * nm, hipMemcpyHostToDevice, s));
-    TTG_HIP(hipMemcpyAsync(mid->d_off, soffv.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice, s));
-    TTG_HIP(hipStreamSynchronize(s));
+    TTG_TRY(h2d_vec(mid->d_qi, sqi));
+    TTG_TRY(h2d_vec(mid->d_off, soffv));
   }
   int* d_mid = up(sc, midnum);
   int* d_ub0 = up(sc, ublk0);
@@ -1770,7 +1771,7 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   }
   hipLaunchKernelGGL(k_svd_scatter, dim3(nQ, 64), dim3(256), 0, s, t, nQ, newdim, d_ub0, d_vb0, Um->d_off,
                      Vm->d_off, Umat, d_umo, Wmat, d_wmo, U->d_data, V->d_data);
-  TTG_HIP(hipStreamSynchronize(s));
+  TTG_HIP(hipGetLastError());   // the outputs are written in stream order; the scratch returns to the pool
   *Uout = Uown.release();
   *Vout = Vown.release();
   *Sout = Sown.release();

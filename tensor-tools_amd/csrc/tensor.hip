@@ -40,10 +40,7 @@ bool TMeta::same_indices(const TMeta& o) const {
 static int upload_idx(TMeta& m) {
   m.d_idx = dalloc<int>(m.h_idx.size());
   TTG_REQUIRE(m.d_idx, "meta: device allocation failed");
-  if (!m.h_idx.empty())
-    TTG_HIP(hipMemcpyAsync(m.d_idx, m.h_idx.data(), m.h_idx.size() * sizeof(int),
-                           hipMemcpyHostToDevice, stream()));
-  return 0;
+  return h2d_vec(m.d_idx, m.h_idx);
 }
 
 int meta_shell(int rank, const int* arrows, const int* nsec, const int* qn, const int* qdim,
@@ -144,12 +141,13 @@ bool same_meta(const MetaPtr& a, const MetaPtr& b) {
   if (x.nblocks != y.nblocks || x.total != y.total || !x.same_indices(y)) return false;
   // compare block lists on the host (small)
   std::vector<int> qa((size_t)x.nblocks * x.rank), qb(qa.size());
-  if (!qa.empty() &&
-      (hipMemcpyAsync(qa.data(), x.d_qi, qa.size() * sizeof(int), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
-       hipMemcpyAsync(qb.data(), y.d_qi, qb.size() * sizeof(int), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
-       hipStreamSynchronize(stream()) != hipSuccess)) {
-    set_error("block list comparison: device copy failed");
-    return false;
+  if (!qa.empty()) {
+    const D2HPiece pc[2] = {{qa.data(), x.d_qi, qa.size() * sizeof(int)},
+                            {qb.data(), y.d_qi, qb.size() * sizeof(int)}};
+    if (d2h_wait(pc, 2) != 0) {
+      set_error("block list comparison: device copy failed");
+      return false;
+    }
   }
   return qa == qb;
 }
@@ -357,14 +355,9 @@ int ttg_tensor_create(int rank, const int* arrows, const int* nsec, const int* q
     off[b + 1] = off[b] + sz;
   }
   TTG_TRY(meta_alloc_blocks(*m, nblocks, (size_t)off[nblocks]));
-  if (nblocks) {
-    TTG_HIP(hipMemcpyAsync(m->d_qi, block_qi, sizeof(int) * (size_t)nblocks * rank,
-                           hipMemcpyHostToDevice, stream()));
-  }
-  TTG_HIP(hipMemcpyAsync(m->d_off, off.data(), sizeof(int64_t) * off.size(),
-                         hipMemcpyHostToDevice, stream()));
+  if (nblocks) TTG_TRY(h2d(m->d_qi, block_qi, sizeof(int) * (size_t)nblocks * rank));
+  TTG_TRY(h2d_vec(m->d_off, off));
   TTG_TRY(tensor_from_meta(m, true, out));
-  TTG_HIP(hipStreamSynchronize(stream()));   // host vector `off` goes out of scope
   return 0;
 }
 
@@ -384,8 +377,10 @@ int ttg_tensor_init_blocks(int rank, const int* arrows, const int* nsec, const i
   int64_t h[2];
   hipLaunchKernelGGL(k_init_blocks, dim3(1), dim3(1024), 0, stream(), m->view(), ncomb, 0,
                      (int*)nullptr, (int64_t*)nullptr, hdr);
-  TTG_HIP(hipMemcpyAsync(h, hdr, sizeof(h), hipMemcpyDeviceToHost, stream()));
-  TTG_HIP(hipStreamSynchronize(stream()));
+  {
+    const D2HPiece pc{h, hdr, sizeof(h)};
+    TTG_TRY(d2h_wait(&pc, 1));
+  }
   TTG_REQUIRE(h[0] < (1 << 30), "initBlock: too many blocks");
   TTG_TRY(meta_alloc_blocks(*m, (int)h[0], (size_t)h[1]));
   hipLaunchKernelGGL(k_init_blocks, dim3(1), dim3(1024), 0, stream(), m->view(), ncomb, 1,

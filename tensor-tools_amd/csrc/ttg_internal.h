@@ -71,6 +71,22 @@ inline hipStream_t stream() { return rt().stream; }
 int side_fork(hipStream_t* side);
 // the main stream waits for everything queued on the side stream so far
 int side_join();
+// host-to-device copy on the library stream through pinned staging: returns
+// once the bytes are staged (the source may be released), the copy itself is
+// ordered on the stream
+int h2d(void* dst, const void* src, size_t bytes);
+template <class T>
+int h2d_vec(T* dst, const std::vector<T>& v) {
+  return h2d(dst, v.data(), sizeof(T) * v.size());
+}
+// device-to-host copies of several ranges into pinned staging, one wait for
+// the library stream, then into the destinations
+struct D2HPiece {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+int d2h_wait(const D2HPiece* pieces, int n);
 
 // caching device pool (size classes; buffers go back to the pool, reuse is
 // ordered by the library's single stream)
