@@ -23,6 +23,7 @@
 //      slicing of U, V and S into blocks.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 #include "ttg_device.h"
@@ -743,6 +744,24 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
   }
 }
 
+// end of an outer sweep: a sector stays active while its sweep rotated
+// anything; the flags start over for the next sweep; *any = some sector active
+__global__ void __launch_bounds__(256) k_bjac_sweep_end(int nsec, int* flag, int* act, int* any) {
+  __shared__ int s_any;
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  int a = 0;
+  for (int li = threadIdx.x; li < nsec; li += blockDim.x) {
+    const int f = flag[li];
+    act[li] = f;
+    flag[li] = 0;
+    a |= f;
+  }
+  if (a) atomicOr(&s_any, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) *any = s_any;
+}
+
 __global__ void k_bjac_finish(const BJSector* sec, int nsec, const int64_t* goff,
                               const int64_t* eoff, const double* Gp, const double* Vp,
                               double* Vout, double* evals) {
@@ -793,6 +812,7 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   int* act = sc.get<int>(ns);
   int* flag = sc.get<int>(ns);
   int* rot = sc.get<int>(pitems.size());
+  int* anyd = sc.get<int>(1);
   TTG_TRY(sc.ok("svd: block Jacobi"));
   std::vector<int> hact(ns, 1);
   TTG_TRY(h2d_vec(d_sec, sec));
@@ -805,22 +825,31 @@ static int block_jacobi(const std::vector<int>& big, const std::vector<int>& nq,
   if (floor_rel > 0)
     hipLaunchKernelGGL(k_bjac_floor, dim3(ns), dim3(256), 0, s, d_sec, ns, d_goff, G, floor_rel);
   hipLaunchKernelGGL(k_bjac_init, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, G, Gp, Vp);
-  for (int sweep = 0; sweep < max_sweeps; sweep++) {
-    TTG_HIP(hipMemsetAsync(flag, 0, sizeof(int) * ns, s));
+  // Outer sweeps.  The active flags move on the device (k_bjac_sweep_end),
+  // so the host reads the "any sector active" word only from the sweep at
+  // which the previous call of the same kind (LR-reduced or plain Gram
+  // matrices) converged, less two: a sweep after convergence finds every
+  // sector inactive and its launches return at once.
+  TTG_HIP(hipMemsetAsync(flag, 0, sizeof(int) * ns, s));
+  static int prev_sweeps[2] = {0, 0};
+  int& prev = prev_sweeps[floor_rel > 0 ? 1 : 0];
+  const int first_check = std::max(0, prev - 2);
+  int sweep = 0;
+  for (; sweep < max_sweeps; sweep++) {
     for (int r = 0; r < maxr; r++) {
       hipLaunchKernelGGL(k_bjac_solve, dim3((unsigned)pitems.size()), dim3(kSolveThreads), kSolveLds, s, d_sec, d_pit,
                          (int)pitems.size(), r, act, flag, rot, Gp, Qp);
       hipLaunchKernelGGL(k_bjac_apply, dim3((unsigned)titems.size()), dim3(256), 0, s, d_sec, d_tit,
                          (int)titems.size(), r, act, rot, Gp, Vp, Qp);
     }
-    std::vector<int> hf(ns);
-    const D2HPiece pc{hf.data(), flag, sizeof(int) * ns};
-    TTG_TRY(d2h_wait(&pc, 1));
+    hipLaunchKernelGGL(k_bjac_sweep_end, dim3(1), dim3(256), 0, s, ns, flag, act, anyd);
+    if (sweep < first_check && sweep + 1 < max_sweeps) continue;
     int any = 0;
-    for (int li = 0; li < ns; li++) any |= hf[li];
+    const D2HPiece pc{&any, anyd, sizeof(int)};
+    TTG_TRY(d2h_wait(&pc, 1));
     if (!any) break;
-    TTG_TRY(h2d_vec(act, hf));
   }
+  prev = std::min(sweep, 16);
   hipLaunchKernelGGL(k_bjac_finish, dim3(64, ns), dim3(256), 0, s, d_sec, ns, d_goff, d_eoff, Gp, Vp,
                      Vev, ev);
   TTG_HIP(hipGetLastError());
@@ -1354,16 +1383,31 @@ static GemmTask gtask(int64_t a_off, int64_t b_off, int em0, int64_t sam0, int e
   return g;
 }
 
-// device copy of a host vector (nullptr, with the error set, on failure)
-template <class T>
-static T* up(Scratch& sc, const std::vector<T>& v) {
-  T* d = sc.get<T>(std::max<size_t>(v.size(), 1));
-  if (d && !v.empty() && h2d_vec(d, v) != 0) {
-    sc.hold(nullptr);   // marks the group failed
-    return nullptr;
+// host arrays uploaded as one copy into one scratch allocation: add() the
+// arrays, upload(), then at<T>(offset) gives each array's device address
+class Pack {
+ public:
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    const size_t off = (bytes_.size() + 15) & ~size_t(15);
+    bytes_.resize(off + sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (!v.empty()) memcpy(bytes_.data() + off, v.data(), sizeof(T) * v.size());
+    return off;
   }
-  return d;
-}
+  int upload(Scratch& sc, const char* what) {
+    d_ = sc.get<char>(std::max<size_t>(bytes_.size(), 1));
+    TTG_TRY(sc.ok(what));
+    return h2d(d_, bytes_.data(), bytes_.size());
+  }
+  template <class T>
+  T* at(size_t off) const {
+    return (T*)(d_ + off);
+  }
+
+ private:
+  std::vector<char> bytes_;
+  char* d_ = nullptr;
+};
 
 // eigenpairs of the large sectors `big` through the LR step (see 4c)
 static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, const std::vector<int64_t>& goff,
@@ -1381,7 +1425,7 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
     nmax = std::max(nmax, nq[q]);
   }
   // panel rows: nb rows of n doubles, d and the flags in the 160 KiB of LDS
-  const int nb = std::max(1, std::min(16, (int)(163840 / (8 * (int64_t)nmax)) - 2));
+  const int nb = std::max(1, std::min(32, (int)(163840 / (8 * (int64_t)nmax)) - 2));
   const size_t lds = sizeof(double) * ((size_t)nb * nmax + nmax) + sizeof(int) * nmax;
   TTG_REQUIRE(lds <= 163840, "svd: sector too large for the pivoted Cholesky panel");
   Scratch sc;
@@ -1426,11 +1470,14 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
   double* W2 = sc.get<double>(std::max<int64_t>(1, goff2[ns]));
   double* gs2 = sc.get<double>(std::max<int64_t>(1, poff2[ns]));
   double* ev2 = sc.get<double>(std::max<int64_t>(1, eoff2[ns]));
-  int* d_r2 = up(sc, r2);
-  int64_t* d_goff2 = up(sc, goff2);
-  int64_t* d_poff2 = up(sc, poff2);
-  int64_t* d_eoff2 = up(sc, eoff2);
   TTG_TRY(sc.ok("svd: LR step"));
+  Pack pk;
+  const size_t o_r2 = pk.add(r2), o_goff2 = pk.add(goff2), o_poff2 = pk.add(poff2), o_eoff2 = pk.add(eoff2);
+  TTG_TRY(pk.upload(sc, "svd: LR step"));
+  int* d_r2 = pk.at<int>(o_r2);
+  int64_t* d_goff2 = pk.at<int64_t>(o_goff2);
+  int64_t* d_poff2 = pk.at<int64_t>(o_poff2);
+  int64_t* d_eoff2 = pk.at<int64_t>(o_eoff2);
   // G' = R R^T: A(k, j) = R[k n + j], B(j, k') = R[k' n + j]
   {
     HostGemm hg;
@@ -1497,35 +1544,48 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   t.uL = sc.get<uint64_t>(nb); t.uR = sc.get<uint64_t>(nb);
   t.lsz = sc.get<int64_t>(nb); t.loff = sc.get<int64_t>(nb);
   t.rsz = sc.get<int64_t>(nb); t.roff = sc.get<int64_t>(nb);
-  t.qL0 = sc.get<int>(nb); t.qL1 = sc.get<int>(nb); t.qR0 = sc.get<int>(nb); t.qR1 = sc.get<int>(nb);
-  t.rows = sc.get<int64_t>(nb); t.cols = sc.get<int64_t>(nb);
-  t.qc = sc.get<uint32_t>(nb);
-  t.hdr = sc.get<int64_t>(4);
+  // the header and the per-sector tables the host reads, in one allocation
+  // (read back with one copy): hdr[4], rows, cols (int64), qc, qL0, qL1,
+  // qR0, qR1 (32-bit), nb entries each
+  const size_t nbp = ((size_t)nb + 31) & ~size_t(31);
+  const size_t tab_bytes = 32 + nbp * (2 * sizeof(int64_t) + 5 * sizeof(int));
+  char* tabs = sc.get<char>(tab_bytes);
   TTG_TRY(sc.ok("svd"));
+  t.hdr = (int64_t*)tabs;
+  t.rows = (int64_t*)(tabs + 32);
+  t.cols = t.rows + nbp;
+  t.qc = (uint32_t*)(t.cols + nbp);
+  t.qL0 = (int*)(t.qc + nbp);
+  t.qL1 = t.qL0 + nbp;
+  t.qR0 = t.qL1 + nbp;
+  t.qR1 = t.qR0 + nbp;
   const unsigned g1 = (nb + 255) / 256;
   hipLaunchKernelGGL(k_svd_keys, dim3(g1), dim3(256), 0, s, m.view(), nleft, kl, kr, ids);
   hipLaunchKernelGGL(k_sort_keys, dim3(g1), dim3(256), 0, s, kl, ids, nb, sl, sid);
   hipLaunchKernelGGL(k_sort_keys, dim3(g1), dim3(256), 0, s, kr, ids, nb, sr, sid);
   hipLaunchKernelGGL(k_svd_unique, dim3(1), dim3(1024), 0, s, m.view(), nleft, sl, sr, nb, t);
-  int64_t hdr[4] = {0, 0, 0, 0};
+  std::vector<char> htab(tab_bytes);
   {
-    const D2HPiece pc{hdr, t.hdr, sizeof(int64_t) * 3};
+    const D2HPiece pc{htab.data(), tabs, tab_bytes};
     TTG_TRY(d2h_wait(&pc, 1));
   }
+  int64_t hdr[4];
+  memcpy(hdr, htab.data(), sizeof(hdr));
   const int nuL = (int)hdr[0], nuR = (int)hdr[1], nQ = (int)hdr[2];
+  TTG_REQUIRE(nQ >= 0 && nQ <= nb, "svd: bad sector count");
+  auto tab = [&](const void* dptr, auto* dst, size_t elem) {
+    memcpy(dst, htab.data() + ((const char*)dptr - tabs), elem * nQ);
+  };
   std::vector<int64_t> rows(nQ), cols(nQ);
   std::vector<uint32_t> qcs(nQ);
   std::vector<int> qL0(nQ), qL1(nQ), qR0(nQ), qR1(nQ);
-  {
-    const D2HPiece pc[7] = {{rows.data(), t.rows, sizeof(int64_t) * nQ},
-                            {cols.data(), t.cols, sizeof(int64_t) * nQ},
-                            {qcs.data(), t.qc, sizeof(uint32_t) * nQ},
-                            {qL0.data(), t.qL0, sizeof(int) * nQ},
-                            {qL1.data(), t.qL1, sizeof(int) * nQ},
-                            {qR0.data(), t.qR0, sizeof(int) * nQ},
-                            {qR1.data(), t.qR1, sizeof(int) * nQ}};
-    TTG_TRY(d2h_wait(pc, 7));
-  }
+  tab(t.rows, rows.data(), sizeof(int64_t));
+  tab(t.cols, cols.data(), sizeof(int64_t));
+  tab(t.qc, qcs.data(), sizeof(uint32_t));
+  tab(t.qL0, qL0.data(), sizeof(int));
+  tab(t.qL1, qL1.data(), sizeof(int));
+  tab(t.qR0, qR0.data(), sizeof(int));
+  tab(t.qR1, qR1.data(), sizeof(int));
   // ---- host layout of the per-sector work arrays
   const bool fromLeft = direction == TTG_MOVE_FROM_LEFT;
   std::vector<int64_t> moff(nQ + 1, 0), goff(nQ + 1, 0), poff(nQ + 1, 0), eoff(nQ + 1, 0);
@@ -1560,15 +1620,19 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   double* ssorted = sc.get<double>(soff[nQ]);
   int* newdim = sc.get<int>(nQ);
   double* tout = sc.get<double>(2);
-  int64_t* d_moff = up(sc, moff);
-  int64_t* d_goff = up(sc, goff);
-  int64_t* d_poff = up(sc, poff);
-  int64_t* d_eoff = up(sc, eoff);
-  int64_t* d_soff = up(sc, soff);
-  int64_t* d_selo = up(sc, selo);
-  int* d_nq = up(sc, nq);
-  int* d_kq = up(sc, kq);
   TTG_TRY(sc.ok("svd"));
+  Pack pk;
+  const size_t o_moff = pk.add(moff), o_goff = pk.add(goff), o_poff = pk.add(poff), o_eoff = pk.add(eoff),
+               o_soff = pk.add(soff), o_selo = pk.add(selo), o_nq = pk.add(nq), o_kq = pk.add(kq);
+  TTG_TRY(pk.upload(sc, "svd"));
+  int64_t* d_moff = pk.at<int64_t>(o_moff);
+  int64_t* d_goff = pk.at<int64_t>(o_goff);
+  int64_t* d_poff = pk.at<int64_t>(o_poff);
+  int64_t* d_eoff = pk.at<int64_t>(o_eoff);
+  int64_t* d_soff = pk.at<int64_t>(o_soff);
+  int64_t* d_selo = pk.at<int64_t>(o_selo);
+  int* d_nq = pk.at<int>(o_nq);
+  int* d_kq = pk.at<int>(o_kq);
   // ---- 2. gather
   fill_f64(mat, moff[nQ], 0.0);
   hipLaunchKernelGGL(k_svd_gather, dim3(std::min(nb, 4096)), dim3(256), 0, s, m.view(), nleft,
@@ -1719,11 +1783,13 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
     TTG_TRY(h2d_vec(mid->d_qi, sqi));
     TTG_TRY(h2d_vec(mid->d_off, soffv));
   }
-  int* d_mid = up(sc, midnum);
-  int* d_ub0 = up(sc, ublk0);
-  int* d_vb0 = up(sc, vblk0);
-  int64_t* d_sout = up(sc, sout);
-  TTG_TRY(sc.ok("svd"));
+  Pack pko;
+  const size_t o_mid = pko.add(midnum), o_ub0 = pko.add(ublk0), o_vb0 = pko.add(vblk0), o_sout = pko.add(sout);
+  TTG_TRY(pko.upload(sc, "svd"));
+  int* d_mid = pko.at<int>(o_mid);
+  int* d_ub0 = pko.at<int>(o_ub0);
+  int* d_vb0 = pko.at<int>(o_vb0);
+  int64_t* d_sout = pko.at<int64_t>(o_sout);
   hipLaunchKernelGGL(k_svd_emit, dim3(1), dim3(1024), 0, s, m.view(), nleft, t, nQ, newdim, d_mid,
                      Um->d_qi, Um->d_off, Vm->d_qi, Vm->d_off);
   ttg_tensor U, V, Sv;
@@ -1743,9 +1809,12 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   }
   double* Umat = sc.get<double>(umo[nQ]);
   double* Wmat = sc.get<double>(wmo[nQ]);
-  int64_t* d_umo = up(sc, umo);
-  int64_t* d_wmo = up(sc, wmo);
   TTG_TRY(sc.ok("svd"));
+  Pack pkm;
+  const size_t o_umo = pkm.add(umo), o_wmo = pkm.add(wmo);
+  TTG_TRY(pkm.upload(sc, "svd"));
+  int64_t* d_umo = pkm.at<int64_t>(o_umo);
+  int64_t* d_wmo = pkm.at<int64_t>(o_wmo);
   {
     HostGemm hg;
     for (int q = 0; q < nQ; q++) {
