@@ -504,7 +504,7 @@ __device__ inline void jac_rotation(double app, double aqq, double apq, double& 
 //      column pairs k' = t%8 + 8u) and Q^T <- M Q^T (the same rows).
 constexpr int kSP = 72;                    // LDS row pitch (doubles) of S and Q^T
 constexpr int kSlotMat = kSub * kSP;
-constexpr size_t kSolveLds = 4 * kSlotMat * sizeof(double) + (kSub + 1) * sizeof(int);
+constexpr size_t kSolveLds = 4 * kSlotMat * sizeof(double) + (2 * kSub + 1) * sizeof(int);
 __device__ inline int slot_of(int x) { return x < kBJ ? 2 * x : 2 * (kSub - 1 - x) + 1; }
 __device__ inline int slot_next(int p) {
   const int k = p >> 1;
@@ -519,33 +519,45 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
              int* rotated, const double* Gp, double* Qp) {
   extern __shared__ __attribute__((aligned(16))) double bj_lds[];
   int* perm = (int*)(bj_lds + 4 * kSlotMat);
-  int* sig = perm + kSub;
+  int* cnt = perm + kSub;
+  int* sig = cnt + kSub;
   const int2 it = items[blockIdx.x];
   const int li = it.x, k = it.y;
   const BJSector S = sec[li];
   if (!active[li] || r >= S.nbk - 1) return;
   const int tid = threadIdx.x;
+  // the pair's two blocks (uniform): subproblem entry t is global index
+  // (t < 32 ? ba : bb) * 32 + t % 32; the sector's own indices (global index
+  // < n) are entries [0, na) and [32, 32 + nbb)
+  int ba, bb;
+  circle_pair(S.nbk, r, k, ba, bb);
+  const int na = min(max(S.n - kBJ * ba, 0), kBJ), nbb = min(max(S.n - kBJ * bb, 0), kBJ);
+  auto gidx = [&](int t) { return (t >= kBJ ? bb : ba) * kBJ + (t & (kBJ - 1)); };
+  auto own = [&](int t) { return t < kBJ ? t < na : t - kBJ < nbb; };
   if (tid == 0) *sig = 0;
+  if (tid < kSub) cnt[tid] = 0;
   {
     double* S0 = bj_lds;
     double* Q0 = bj_lds + 2 * kSlotMat;
     for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
       const int i = e % kSub, j = e / kSub;   // G is exactly symmetric
-      const int gi = bj_index(S.nbk, r, k, i), gj = bj_index(S.nbk, r, k, j);
-      S0[slot_of(i) * kSP + slot_of(j)] = Gp[S.gp + gi + (int64_t)S.N * gj];
+      S0[slot_of(i) * kSP + slot_of(j)] = Gp[S.gp + gidx(i) + (int64_t)S.N * gidx(j)];
       Q0[slot_of(j) * kSP + i] = i == j ? 1.0 : 0.0;
     }
   }
   __syncthreads();
   {
+    // significance test of this subproblem (before rotating):
+    // |v| > 1e-12 sqrt(g_pp g_qq), compared squared; thread t's rows p are
+    // all t % 64 (so one diagonal load), its columns q = t / 64 + 4 i
     const double* S0 = bj_lds;
-    bool any = false;   // significance test of this subproblem (before rotating)
-    for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
-      const int p = e % kSub, q = e / kSub;
+    const int p = tid % kSub;
+    const double gpp = S0[p * kSP + p];
+    bool any = false;
+    for (int q = tid / kSub; q < kSub; q += kSolveThreads / kSub) {
       if (p == q) continue;
       const double v = S0[p * kSP + q];
-      const double dd = sqrt(fabs(S0[p * kSP + p] * S0[q * kSP + q]));
-      any |= fabs(v) > kBJRel * dd && fabs(v) > S.floor;
+      any |= v * v > (kBJRel * kBJRel) * fabs(gpp * S0[q * kSP + q]) && fabs(v) > S.floor;
     }
     if (any) *sig = 1;
   }
@@ -620,24 +632,34 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
   // eigenvector moves into the padding that k_bjac_finish drops.
   const double* Sf = bj_lds + kSlotMat;
   const double* Qf = bj_lds + 3 * kSlotMat;
-  if (tid < kSub) {
-    const int x = tid, p = slot_of(x);
-    int dst = x;
-    if (bj_index(S.nbk, r, k, x) < S.n) {
-      const double d = Sf[p * kSP + p];
-      int rk = 0;
-      for (int u = 0; u < kSub; u++) {
-        if (bj_index(S.nbk, r, k, u) >= S.n) continue;
+  {
+    // rank of own entry x among the own entries (descending eigenvalue,
+    // ties to the lower entry), counted by the four waves over a quarter of
+    // the entries each
+    const int x = tid % kSub, part = tid / kSub;
+    if (own(x)) {
+      const double d = Sf[slot_of(x) * kSP + slot_of(x)];
+      int c = 0;
+      for (int u = 16 * part; u < 16 * part + 16; u++) {
+        if (!own(u)) continue;
         const int pu = slot_of(u);
         const double du = Sf[pu * kSP + pu];
-        rk += (du > d) || (du == d && u < x);
+        c += (du > d) || (du == d && u < x);
       }
-      for (int u = 0; u < kSub; u++) {   // the rk-th own index in ascending order
-        if (bj_index(S.nbk, r, k, u) >= S.n) continue;
-        if (rk-- == 0) { dst = u; break; }
-      }
+      if (c) atomicAdd(&cnt[x], c);
     }
-    perm[p] = dst;   // column of Q for the eigenvector in slot p
+  }
+  __syncthreads();
+  if (tid < kSub) {
+    // the rank-th own entry in ascending order: own entries are [0, na) then
+    // [32, 32 + nbb)
+    const int x = tid;
+    int dst = x;
+    if (own(x)) {
+      const int rk = cnt[x];
+      dst = rk < na ? rk : kBJ + (rk - na);
+    }
+    perm[slot_of(x)] = dst;   // column of Q for the eigenvector in slot slot_of(x)
   }
   __syncthreads();
   for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
