@@ -83,7 +83,22 @@ def sweep_leg(P, T, args):
         r = P.dmrg(psi, H, args.sweeps, [args.sweep_m], [args.sweep_cutoff], [2], tol=1e-12)
         hs = np.asarray(r["halfsweep_s"], dtype=float)
         sweep_s = float(np.sum(hs)) / args.sweeps
-        res[name] = dict(sweep_s=round(sweep_s, 4),
+        warm = None
+        if args.sweep_warm and args.sweep_m <= 4096:
+            # the same sweeps again from the same state: the same work, with the
+            # tile-shape choices the first run timed already cached (a DMRG run's
+            # later sweeps find most of them there)
+            del psi
+            psi = P.mps_product([i % 2 for i in range(N)])
+            for i, t in enumerate(tensors):
+                psi.set_tensor(i, P.PTensor.from_flat(t))
+            psi.set_center(-1)
+            r2 = P.dmrg(psi, H, args.sweeps, [args.sweep_m], [args.sweep_cutoff], [2], tol=1e-12)
+            warm = round(float(np.sum(r2["halfsweep_s"])) / args.sweeps, 4)
+            if r2["E"] != r["E"]:
+                log(f"[bench] note: {name} repeat sweep energy differs by {abs(r2['E'] - r['E']):.2e} "
+                    f"(tile shapes chosen by timing differ between the runs)")
+        res[name] = dict(sweep_s=round(sweep_s, 4), sweep_s_warm=warm,
                          sweep_s_each=[round(float(v), 4) for v in hs.reshape(-1, 2).sum(axis=1)],
                          halfsweep_s=[round(float(v), 4) for v in hs],
                          total_s=round(r["seconds"], 4), E=r["E"], E_per_site=r["E"] / N,
@@ -99,7 +114,7 @@ def sweep_leg(P, T, args):
             with open(args.env_log, "a") as f:
                 f.write(json.dumps({"form": name, "m": args.sweep_m, "workload": args.workload,
                                     "env_elems": [int(v) for v in r["env_elems"]]}) + "\n")
-        log(f"[bench] sweep {name}: {sweep_s:.3f} s (halves {res[name]['halfsweep_s']}, "
+        log(f"[bench] sweep {name}: {sweep_s:.3f} s, repeated {warm} s (halves {res[name]['halfsweep_s']}, "
             f"davidson {r['t_davidson']:.2f} s, svd {r['t_svd']:.2f} s, env {r['t_env']:.2f} s, "
             f"{r['n_matvec']} matvecs), E/N = {r['E'] / N:.12f}, max bond {res[name]['max_bond']}")
         del psi
@@ -239,6 +254,9 @@ def main():
                     help="bond dimension of the DMRG sweep leg (0 skips the sweep)")
     ap.add_argument("--sweep-forms", default="fp64,bf16x3,fp32")
     ap.add_argument("--sweeps", type=int, default=1, help="sweeps of the sweep leg (sweep_s is per sweep)")
+    ap.add_argument("--sweep-warm", type=int, default=1,
+                    help="1: run the sweep leg a second time from the same state and report its time "
+                         "(sweep_s_warm: tile-shape choices already timed); m <= 4096 only")
     ap.add_argument("--sweep-cutoff", type=float, default=1e-9, help="SVD cutoff of the sweep leg")
     ap.add_argument("--env-stride", type=int, default=0,
                     help="keep every k-th environment between half-sweeps (0: 1 up to m=4096, else 4)")

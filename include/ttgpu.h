@@ -120,6 +120,12 @@ int ttg_contract(ttg_tensor A, ttg_tensor B, int npairs, const int* pa, const in
  * as one captured HIP graph per operator. */
 int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_like,
                     const int* pairs3, int prec, ttg_heff* out);
+/* the same operator at another precision, on src's plans (output blocks and
+ * task lists copied on the device, no QN bookkeeping) and src's tile shapes;
+ * src's L, W12 and R are read by both.  The FP64 twin serves the residual
+ * correction of the reduced forms (tensor_davidson.cpp:255-276 computes its
+ * products with the one FP64 operator) */
+int ttg_heff_create_twin(ttg_heff src, int prec, ttg_heff* out);
 int ttg_heff_new_output(ttg_heff h, ttg_tensor* y);
 int ttg_heff_apply(ttg_heff h, ttg_tensor x, ttg_tensor y);
 /* algorithmic FLOPs of one apply (sum of 2*M*N*K over the 3 stages) and the

@@ -186,6 +186,57 @@ static int to_f32(ttg_tensor t, float** out) {
   return 0;
 }
 
+// intermediates, FP32 operand copies (reduced forms), the measured tile
+// shapes when `tune`, and the captured apply graph of an operator whose three
+// plans exist; 1 with the error set on failure (the caller deletes h)
+static int heff_finish(ttg_heff h, bool tune) {
+  const bool red = h->prec != TTG_FP64;
+  const size_t es = red ? sizeof(float) : sizeof(double);
+  h->T1 = dmalloc(std::max<size_t>(1, h->p[0]->C->total) * es);
+  h->T2 = dmalloc(std::max<size_t>(1, h->p[1]->C->total) * es);
+  h->xbuf = dalloc<double>(h->xm->total);
+  h->ybuf = dalloc<double>(h->xm->total);
+  if (!h->T1 || !h->T2 || !h->xbuf || !h->ybuf) {
+    set_error("heff: allocation failed");
+    return 1;
+  }
+  fill_f64(h->ybuf, h->xm->total, 0.0);   // blocks of x never produced stay zero
+  if (red) {
+    h->x32 = dalloc<float>(h->xm->total);
+    if (!h->x32 || to_f32(h->L, &h->L32) || to_f32(h->W, &h->W32) || to_f32(h->R, &h->R32)) {
+      set_error("heff: allocation failed");
+      return 1;
+    }
+  }
+  hipStream_t s = stream();
+  if (tune) {   // measured tile shapes of the MFMA stages (before the graph is captured)
+    const void* a1 = red ? (const void*)h->L32 : (const void*)h->L->d_data;
+    const void* b1 = red ? (const void*)h->x32 : (const void*)h->xbuf;
+    const void* b2 = red ? (const void*)h->W32 : (const void*)h->W->d_data;
+    const void* b3 = red ? (const void*)h->R32 : (const void*)h->R->d_data;
+    if (tune_stage(h->p[0], 0, a1, b1, h->T1, !red, s) || tune_stage(h->p[1], 1, h->T1, b2, h->T2, !red, s) ||
+        tune_stage(h->p[2], 2, h->T2, b3, h->ybuf, true, s))
+      return 1;
+  }
+  // capture the apply sequence once
+  TTG_HIP(hipStreamSynchronize(s));
+  TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rr = record(h, s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rr || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    set_error("heff: graph capture failed");
+    return 1;
+  }
+  h->graph = g;
+  {
+    PhaseTimer _inst("heff_create: graph instantiate");
+    TTG_HIP(hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0));
+  }
+  return 0;
+}
+
 }  // namespace ttg
 
 using namespace ttg;
@@ -219,51 +270,34 @@ int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_lik
     set_error("heff: L*x*W12*R does not reproduce x's index sectors");
     return 1;
   }
-  if (remap_to_x(h->p[2], h->xm)) { delete h; return 1; }
-  const bool red = prec != TTG_FP64;
-  const size_t es = red ? sizeof(float) : sizeof(double);
-  h->T1 = dmalloc(std::max<size_t>(1, h->p[0]->C->total) * es);
-  h->T2 = dmalloc(std::max<size_t>(1, h->p[1]->C->total) * es);
-  h->xbuf = dalloc<double>(h->xm->total);
-  h->ybuf = dalloc<double>(h->xm->total);
-  if (!h->T1 || !h->T2 || !h->xbuf || !h->ybuf) { delete h; set_error("heff: allocation failed"); return 1; }
-  fill_f64(h->ybuf, h->xm->total, 0.0);   // blocks of x never produced stay zero
-  if (red) {
-    h->x32 = dalloc<float>(h->xm->total);
-    if (!h->x32 || to_f32(L, &h->L32) || to_f32(W12, &h->W32) || to_f32(R, &h->R32)) {
-      delete h;
-      set_error("heff: allocation failed");
-      return 1;
-    }
-  }
-  hipStream_t s = stream();
-  {   // measured tile shapes of the MFMA stages (before the graph is captured)
-    const void* a1 = red ? (const void*)h->L32 : (const void*)L->d_data;
-    const void* b1 = red ? (const void*)h->x32 : (const void*)h->xbuf;
-    const void* b2 = red ? (const void*)h->W32 : (const void*)W12->d_data;
-    const void* b3 = red ? (const void*)h->R32 : (const void*)R->d_data;
-    if (tune_stage(h->p[0], 0, a1, b1, h->T1, !red, s) || tune_stage(h->p[1], 1, h->T1, b2, h->T2, !red, s) ||
-        tune_stage(h->p[2], 2, h->T2, b3, h->ybuf, true, s)) {
-      delete h;
-      return 1;
-    }
-  }
-  // capture the apply sequence once
-  TTG_HIP(hipStreamSynchronize(s));
-  TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rr = record(h, s);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(s, &g);
-  if (rr || e != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
+  if (remap_to_x(h->p[2], h->xm) || heff_finish(h, true)) {
     delete h;
-    set_error("heff: graph capture failed");
     return 1;
   }
-  h->graph = g;
-  {
-    PhaseTimer _inst("heff_create: graph instantiate");
-    TTG_HIP(hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0));
+  *out = h;
+  return 0;
+}
+
+int ttg_heff_create_twin(ttg_heff src, int prec, ttg_heff* out) {
+  PhaseTimer _phase("ttg_heff_create_twin");
+  TTG_REQUIRE(src && out, "heff_create_twin: null argument");
+  TTG_REQUIRE(prec == TTG_FP64 || prec == TTG_FP32 || prec == TTG_BF16X3, "unknown precision");
+  ttg_heff h = new ttg_heff_s();
+  h->prec = prec;
+  h->L = src->L; h->W = src->W; h->R = src->R;
+  h->xm = src->xm;
+  h->flops = src->flops;
+  for (int st = 0; st < 3; st++) {
+    if (plan_clone(*src->p[st], prec, &h->p[st])) {
+      delete h;
+      return 1;
+    }
+  }
+  // the source's tile shapes (a twin serves a few products: timing
+  // candidate shapes for it would cost more than it saves)
+  if (heff_finish(h, false)) {
+    delete h;
+    return 1;
   }
   *out = h;
   return 0;

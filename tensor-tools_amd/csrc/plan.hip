@@ -645,6 +645,56 @@ int rebuild_tiles(ttg_plan_s& p, int cfg) {
   return build_tiles(p, hcb);
 }
 
+// k-tile offsets of every task of a block's k walk, and the block's k-tile
+// count, for k-tile depth TK (one thread per output block)
+__global__ void __launch_bounds__(256) k_plan_ktiles(int nblocks, CBlock* cblk, GemmTask* tasks, int TK) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const CBlock cb = cblk[b];
+  int kt = 0;
+  for (int t = cb.t0; t < cb.t1; t++) {
+    const GemmTask& g = tasks[t];
+    tasks[t].kt0 = kt;
+    kt += (g.K / g.ek0) * ((g.ek0 + TK - 1) / TK);
+  }
+  cblk[b].nkt = kt;
+}
+
+int plan_clone(const ttg_plan_s& src, int prec, ttg_plan* out) {
+  TTG_REQUIRE(prec == TTG_FP64 || prec == TTG_FP32 || prec == TTG_BF16X3, "unknown precision");
+  std::unique_ptr<ttg_plan_s> p(new ttg_plan_s());
+  p->A = src.A;
+  p->B = src.B;
+  p->C = src.C;
+  p->ps = src.ps;
+  p->prec = prec;
+  p->nblocksC = src.nblocksC;
+  p->ntasks = src.ntasks;
+  p->flops = src.flops;
+  p->cfg = src.cfg;
+  p->a_mcontig = src.a_mcontig;
+  p->b_kcontig = src.b_kcontig;
+  const int64_t nt = std::max<int64_t>(1, p->ntasks);
+  const int64_t nb = std::max<int64_t>(1, p->nblocksC);
+  p->d_tasks = dalloc<GemmTask>(nt);
+  p->d_cblk = dalloc<CBlock>(nb);
+  p->d_tuples = dalloc<int>(6 * nt);
+  TTG_REQUIRE(p->d_tasks && p->d_cblk && p->d_tuples, (std::string("plan: ") + ttg_last_error()).c_str());
+  hipStream_t s = stream();
+  TTG_HIP(hipMemcpyAsync(p->d_tasks, src.d_tasks, sizeof(GemmTask) * nt, hipMemcpyDeviceToDevice, s));
+  TTG_HIP(hipMemcpyAsync(p->d_cblk, src.d_cblk, sizeof(CBlock) * nb, hipMemcpyDeviceToDevice, s));
+  TTG_HIP(hipMemcpyAsync(p->d_tuples, src.d_tuples, sizeof(int) * 6 * nt, hipMemcpyDeviceToDevice, s));
+  if (tk_of(prec) != tk_of(src.prec) && p->nblocksC > 0)
+    hipLaunchKernelGGL(k_plan_ktiles, dim3((p->nblocksC + 255) / 256), dim3(256), 0, s, p->nblocksC,
+                       p->d_cblk, p->d_tasks, tk_of(prec));
+  std::vector<CBlock> hcb(p->nblocksC);
+  const D2HPiece pc{hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)p->nblocksC};
+  TTG_TRY(d2h_wait(&pc, 1));
+  TTG_TRY(build_tiles(*p, hcb));
+  *out = p.release();
+  return 0;
+}
+
 // pinned host copy of a plan header, allocated once for the process: plans
 // are built one at a time, and every read of it follows a stream sync
 static PlanHdr* pinned_hdr() {

@@ -118,6 +118,10 @@ int gemm_occupancy(int prec, int cfg, bool a_mcontig, bool b_kcontig);
 int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb);
 // the same plan's work list rebuilt for another MFMA tile shape
 int rebuild_tiles(ttg_plan_s& p, int cfg);
+// a plan for precision prec on src's output blocks and task list (copied on
+// the device, no QN bookkeeping), with src's tile shape: the k-tile offsets
+// are recomputed when prec's k-tile depth differs
+int plan_clone(const ttg_plan_s& src, int prec, ttg_plan* out);
 // measured tile-shape choice of one stage of a multi-stage operator (the
 // operands are the stage's own; C is overwritten); `stage` keys the cache of
 // choices together with the stage's precision, size and operand layout

@@ -284,10 +284,8 @@ void big_qtensor<double>::ensure_plan(const qtensor<double>& v) {
   check(ttg_heff_create(L->dev.get(), mid.dev.get(), R->dev.get(), full.dev.get(), enc.data(),
                         g_heff_prec, &heff),
         "heff_create");
-  if (g_heff_prec != TTG_FP64)
-    check(ttg_heff_create(L->dev.get(), mid.dev.get(), R->dev.get(), full.dev.get(), enc.data(),
-                          TTG_FP64, &heff64),
-          "heff_create (FP64 correction)");
+  if (g_heff_prec != TTG_FP64)   // FP64 twin on the same plans, for the correction products
+    check(ttg_heff_create_twin(heff, TTG_FP64, &heff64), "heff_create (FP64 correction)");
   plan_prec = g_heff_prec;
 }
 

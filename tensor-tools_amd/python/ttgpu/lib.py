@@ -70,6 +70,7 @@ def load(path: str = LIB_PATH):
         "ttg_contract_exec": ([_p, _p, _p, _p], _i), "ttg_plan_free": ([_p], _i),
         "ttg_contract": ([_p, _p, _i, _ip, _ip, _i, C.POINTER(_p)], _i),
         "ttg_heff_create": ([_p, _p, _p, _p, _ip, _i, C.POINTER(_p)], _i),
+        "ttg_heff_create_twin": ([_p, _i, C.POINTER(_p)], _i),
         "ttg_heff_new_output": ([_p, C.POINTER(_p)], _i),
         "ttg_heff_apply": ([_p, _p, _p], _i),
         "ttg_heff_info": ([_p, C.POINTER(_d), _lp, _dp], _i),
@@ -280,6 +281,16 @@ class Heff:
         check(lib().ttg_heff_create(L.h, W12.h, R.h, x_like.h, np.array(enc, np.int32), prec,
                                     C.byref(self.h)))
         self.xidx = x_like.idx
+
+    @classmethod
+    def twin(cls, src: "Heff", prec) -> "Heff":
+        """The same operator at precision `prec` on src's plans and tile shapes."""
+        t = cls.__new__(cls)
+        t._keep = src._keep + (src,)
+        t.h = _p()
+        check(lib().ttg_heff_create_twin(src.h, prec, C.byref(t.h)))
+        t.xidx = src.xidx
+        return t
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -97,3 +97,35 @@ def test_small_eigh_matches_numpy():
         ref = np.linalg.eigvalsh(A)
         assert np.allclose(ev, ref, atol=1e-13)
         assert np.allclose(A @ V, V * ev, atol=1e-12)
+
+
+@pytest.mark.parametrize("src_prec,twin_prec", [(T.BF16X3, T.FP64), (T.FP32, T.FP64), (T.FP64, T.BF16X3)])
+def test_heff_twin_matches_direct_operator(src_prec, twin_prec):
+    """ttg_heff_create_twin: an operator at another precision on the source's
+    plans (block and task lists copied, k-tile offsets recomputed when the
+    k-tile depth differs) gives the oracle's product to the twin precision's
+    bound, and each stage's block list, offsets and task tuples are those of
+    the operator built directly at that precision."""
+    case = build_case("j1j2_chain", 50, 160, 8, 3.0)
+    h, dx, _ = run_heff(case, src_prec)
+    tw = T.Heff.twin(h, twin_prec)
+    y = tw.new_output()
+    tw.apply(dx, y)
+    ref = dense_y(case)
+    err = np.linalg.norm(y.to_flat().to_dense() - ref) / np.linalg.norm(ref)
+    assert err <= TOL[twin_prec], err
+    direct, _, yd = run_heff(case, twin_prec)
+    if twin_prec == T.FP64:
+        d = np.linalg.norm(yd.to_flat().to_dense() - y.to_flat().to_dense()) / np.linalg.norm(ref)
+        assert d <= 1e-13, d
+    Li, xi = T.DTensor.from_flat(case["L"]).idx, dx.idx
+    Wi, Ri = T.DTensor.from_flat(case["W12"]).idx, T.DTensor.from_flat(case["R"]).idx
+    T1i = T.result_indices(Li, xi, case["pairs"][0])
+    T2i = T.result_indices(T1i, Wi, case["pairs"][1])
+    for st, (A_idx, B_idx) in enumerate([(Li, xi), (T1i, Wi), (T2i, Ri)]):
+        pt = tw.stage_plan(st, A_idx, B_idx, case["pairs"][st])
+        pd = direct.stage_plan(st, A_idx, B_idx, case["pairs"][st])
+        assert np.array_equal(pt.tasks(), pd.tasks()), f"stage {st + 1}: task list"
+        qt, ot = pt.new_output().meta()
+        qd, od = pd.new_output().meta()
+        assert np.array_equal(qt, qd) and np.array_equal(ot, od), f"stage {st + 1}: block list"
