@@ -1119,27 +1119,52 @@ __global__ void __launch_bounds__(256) k_svd_gather_sel(const int* nq, const int
 // sequentially in the reference's order; the global order is a stable
 // descending sort (the reference's std::sort is unstable on ties).
 // 5b. global descending order of all singular values (ties by position,
-// a stable sort as the reference's pairCompare sort over (sector, desc) order)
-__global__ void __launch_bounds__(256) k_svd_rank(int T, const double* __restrict__ svals,
-                                                  int* __restrict__ order, double* __restrict__ sorted) {
-  __shared__ double chunk[kRankChunk];
+// a stable sort as the reference's pairCompare sort over (sector, desc) order).
+// Each sector's values are already non-increasing (k_svd_select), so the rank
+// of value v at position j of sector q is j, plus the values >= v of the
+// sectors before q (they come first on ties), plus the values > v of the
+// sectors after it: one binary search per other sector instead of a pass
+// over all values.
+__device__ inline int count_ge(const double* a, int n, double v) {   // a non-increasing
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] >= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ inline int count_gt(const double* a, int n, double v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] > v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__global__ void __launch_bounds__(256) k_svd_rank(int T, int nQ, const int64_t* __restrict__ soff,
+                                                  const double* __restrict__ svals, int* __restrict__ order,
+                                                  double* __restrict__ sorted) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double v = i < T ? rank_key(svals[i]) : 0.0;
-  int r = 0;
-  for (int c0 = 0; c0 < T; c0 += kRankChunk) {   // values through LDS, as in k_svd_select
-    const int cn = min(kRankChunk, T - c0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < cn; e += blockDim.x) chunk[e] = rank_key(svals[c0 + e]);
-    __syncthreads();
-    for (int j = 0; j < cn; j++) {
-      const double u = chunk[j];
-      r += (u > v) || (u == v && c0 + j < i);
-    }
+  if (i >= T) return;
+  const double v = svals[i];
+  int lo = 0, hi = nQ;   // sector q of i: the last q with soff[q] <= i
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (soff[mid] <= i) lo = mid;
+    else hi = mid;
   }
-  if (i < T) {
-    order[r] = i;
-    sorted[r] = svals[i];
+  const int q = lo;
+  int r = i - (int)soff[q];
+  for (int o = 0; o < nQ; o++) {
+    if (o == q) continue;
+    const double* a = svals + soff[o];
+    const int n = (int)(soff[o + 1] - soff[o]);
+    r += o < q ? count_ge(a, n, v) : count_gt(a, n, v);
   }
+  order[r] = i;
+  sorted[r] = v;
 }
 
 // 5c. the reference's keep loop (qtensor_op.cpp:1294-1300), sequential in its
@@ -1740,7 +1765,8 @@ extern "C" int ttg_svd(ttg_tensor A, int nleft, int direction, double cutoff, in
   {
     const int T = (int)soff[nQ];
     if (T > 0)
-      hipLaunchKernelGGL(k_svd_rank, dim3((T + 255) / 256), dim3(256), 0, s, T, svals, order, ssorted);
+      hipLaunchKernelGGL(k_svd_rank, dim3((T + 255) / 256), dim3(256), 0, s, T, nQ, d_soff, svals, order,
+                         ssorted);
     hipLaunchKernelGGL(k_svd_truncate, dim3(1), dim3(256), 0, s, T, nQ, svals, ssec, order, ssorted,
                        cutoff, K, newdim, tout);
   }
