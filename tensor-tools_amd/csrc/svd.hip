@@ -410,12 +410,8 @@ __device__ inline void circle_pair(int m, int r, int k, int& a, int& b) {
   else { a = (r + k) % m1; b = (r - k + m1) % m1; }
   if (a > b) { const int t = a; a = b; b = t; }
 }
-// global index (0..N-1) of subproblem entry t (0..63) for the pair k of round r
-__device__ inline int bj_index(int nbk, int r, int k, int t) {
-  int a, b;
-  circle_pair(nbk, r, k, a, b);
-  return (t >= kBJ ? b : a) * kBJ + (t & (kBJ - 1));
-}
+// subproblem entry t (0..63) of pair k of round r is global index
+// (t < kBJ ? a : b) * kBJ + t % kBJ, (a, b) = circle_pair(nbk, r, k)
 
 // sec[li].dmax = max_i |G_ii| of the sector, sec[li].floor = floor_rel * dmax
 __global__ void __launch_bounds__(256) k_bjac_floor(BJSector* sec, int nsec, const int64_t* goff, const double* G,
@@ -692,6 +688,12 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
   if (!ra && !rb) return;   // both factors are the identity
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int l16 = lane & 15, l4 = lane >> 4;
+  // global indices of the entries of pairs a and b (their blocks computed once)
+  int a0, a1, b0, b1;
+  circle_pair(S.nbk, r, a, a0, a1);
+  circle_pair(S.nbk, r, b, b0, b1);
+  auto ga = [&](int t) { return (t >= kBJ ? a1 : a0) * kBJ + (t & (kBJ - 1)); };
+  auto gb = [&](int t) { return (t >= kBJ ? b1 : b0) * kBJ + (t & (kBJ - 1)); };
   const double* qa = Qp + S.qp + (int64_t)a * kSub * kSub;
   const double* qb = Qp + S.qp + (int64_t)b * kSub * kSub;
   double af[16];
@@ -702,7 +704,7 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
     for (int s = 0; s < 16; s++) af[s] = qb[(4 * s + l4) + kSub * (16 * w + l16)];   // Qb^T
     for (int e = tid; e < kSub * kSub; e += 256) {   // Bs[k][ii] = V(64a + ii, gj(k))
       const int ii = e % kSub, kk = e / kSub;
-      Bs[kk * kTLd + ii] = V[(int64_t)(a * kSub + ii) + (int64_t)S.N * bj_index(S.nbk, r, b, kk)];
+      Bs[kk * kTLd + ii] = V[(int64_t)(a * kSub + ii) + (int64_t)S.N * gb(kk)];
     }
     __syncthreads();
 #pragma unroll
@@ -715,7 +717,7 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
                                                       0, 0, 0);
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-      const int64_t col = (int64_t)S.N * bj_index(S.nbk, r, b, 16 * w + l4 + 4 * rr);
+      const int64_t col = (int64_t)S.N * gb(16 * w + l4 + 4 * rr);
 #pragma unroll
       for (int f = 0; f < 4; f++) V[col + a * kSub + 16 * f + l16] = acc[f][rr];
     }
@@ -726,7 +728,7 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
   for (int s = 0; s < 16; s++) af[s] = qa[(4 * s + l4) + kSub * (16 * w + l16)];   // Qa^T
   for (int e = tid; e < kSub * kSub; e += 256) {
     const int kk = e % kSub, j = e / kSub;
-    Bs[kk * kTLd + j] = G[bj_index(S.nbk, r, a, kk) + (int64_t)S.N * bj_index(S.nbk, r, b, j)];
+    Bs[kk * kTLd + j] = G[ga(kk) + (int64_t)S.N * gb(j)];
     Xs[kk * kTLd + j] = qb[kk + kSub * j];
   }
   __syncthreads();
@@ -756,10 +758,10 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
                                                     0, 0, 0);
 #pragma unroll
   for (int rr = 0; rr < 4; rr++) {
-    const int gi = bj_index(S.nbk, r, a, 16 * w + l4 + 4 * rr);
+    const int gi = ga(16 * w + l4 + 4 * rr);
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-      const int gj = bj_index(S.nbk, r, b, 16 * f + l16);
+      const int gj = gb(16 * f + l16);
       G[gi + (int64_t)S.N * gj] = acc[f][rr];
       if (a != b) G[gj + (int64_t)S.N * gi] = acc[f][rr];   // G stays symmetric
     }
