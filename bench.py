@@ -236,7 +236,7 @@ def cpu_bond_sample(args, N):
     log(f"[bench] cpu bond sample: davidson {t1 - t0:.2f} s, svd {t2 - t1:.2f} s, env {t3 - t2:.2f} s "
         f"on {O.threads()} threads")
     return {"sweep_s_estimate": round(per_bond * 2 * (N - 1), 2), "unit": "s", "cores": O.threads(),
-            "kind": "port", "per_bond_s": round(per_bond, 3),
+            "kind": "port", "blas": O.blas_name(), "per_bond_s": round(per_bond, 3),
             "sample": "one bulk bond update (Davidson, sector SVD, environment step) of the oracle "
                       "at the sweep's m, times 2(N-1) bonds"}
 
@@ -363,9 +363,9 @@ def main():
             num += float(np.sum((ob_ - (gb if gb is not None else 0.0)) ** 2))
             den += float(np.sum(ob_ ** 2))
         cpu = {"value": round(flops / (tn / n) / 1e12, 5), "unit": "TFLOP/s", "cores": O.threads(),
-               "kind": "port",
+               "kind": "port", "blas": O.blas_name(),
                "sample": f"{n} H_eff matvecs of the same workload ({tn / n * 1e3:.1f} ms each, "
-                         f"FP64 oracle, OpenMP over output blocks)",
+                         f"FP64 oracle, OpenMP over output blocks, block products on {O.blas_name()})",
                "fp64_gpu_vs_oracle_relerr": float(np.sqrt(num / den))}
         log(f"[bench] cpu baseline: {tn / n * 1e3:.1f} ms/matvec on {O.threads()} threads; "
             f"GPU FP64 vs oracle rel err {cpu['fp64_gpu_vs_oracle_relerr']:.2e}")
