@@ -211,10 +211,17 @@ k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cb
       const unsigned ua = (unsigned)(((int64_t)w.c * g.sak0 + (int64_t)w.k1 * g.sak1) * ESZ);
       const unsigned ub = (unsigned)(((int64_t)w.c * g.sbk0 + (int64_t)w.k1 * g.sbk1) * ESZ);
       const int klim = g.ek0 - w.c;
+      if (klim >= TK) {   // the whole k-tile lies inside the chunk (uniform): no per-element test
 #pragma unroll
-      for (int i = 0; i < NA; i++) ra[i] = buf_load<T>(rsA, ak[i] < klim ? aoff[i] + ua : kOOB);
+        for (int i = 0; i < NA; i++) ra[i] = buf_load<T>(rsA, aoff[i] + ua);
 #pragma unroll
-      for (int i = 0; i < NB; i++) rb[i] = buf_load<T>(rsB, bk[i] < klim ? boff[i] + ub : kOOB);
+        for (int i = 0; i < NB; i++) rb[i] = buf_load<T>(rsB, boff[i] + ub);
+      } else {
+#pragma unroll
+        for (int i = 0; i < NA; i++) ra[i] = buf_load<T>(rsA, ak[i] < klim ? aoff[i] + ua : kOOB);
+#pragma unroll
+        for (int i = 0; i < NB; i++) rb[i] = buf_load<T>(rsB, bk[i] < klim ? boff[i] + ub : kOOB);
+      }
     };
     auto store = [&](int buf) {
 #pragma unroll
@@ -367,19 +374,30 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
       const unsigned ub = (unsigned)(((int64_t)w.c * g.sbk0 + (int64_t)w.k1 * g.sbk1) * 4);
       const unsigned sa = (unsigned)(g.sak0 * 4), sb = (unsigned)(g.sbk0 * 4);
       const int klim = g.ek0 - w.c;
+      if (klim >= TK) {   // the whole k-tile lies inside the chunk (uniform): no per-element test
 #pragma unroll
-      for (int i = 0; i < GA; i++) {
-        const int kl = klim - 8 * akg[i];
+        for (int i = 0; i < GA; i++)
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-          ra[i][u] = buf_load<float>(rsA, u < kl ? aoff[i] + (ua + u * sa) : kOOB);
-      }
+          for (int u = 0; u < 8; u++) ra[i][u] = buf_load<float>(rsA, aoff[i] + (ua + u * sa));
 #pragma unroll
-      for (int i = 0; i < GB; i++) {
-        const int kl = klim - 8 * bkg[i];
+        for (int i = 0; i < GB; i++)
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-          rb[i][u] = buf_load<float>(rsB, u < kl ? boff[i] + (ub + u * sb) : kOOB);
+          for (int u = 0; u < 8; u++) rb[i][u] = buf_load<float>(rsB, boff[i] + (ub + u * sb));
+      } else {
+#pragma unroll
+        for (int i = 0; i < GA; i++) {
+          const int kl = klim - 8 * akg[i];
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            ra[i][u] = buf_load<float>(rsA, u < kl ? aoff[i] + (ua + u * sa) : kOOB);
+        }
+#pragma unroll
+        for (int i = 0; i < GB; i++) {
+          const int kl = klim - 8 * bkg[i];
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            rb[i][u] = buf_load<float>(rsB, u < kl ? boff[i] + (ub + u * sb) : kOOB);
+        }
       }
     };
     auto store = [&](int buf) {
