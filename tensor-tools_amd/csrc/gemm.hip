@@ -61,6 +61,11 @@ struct MfmaOp<float> {
 __device__ __forceinline__ int64_t off2(int x, int e0, int64_t s0, int64_t s1) {
   return (int64_t)(x % e0) * s0 + (int64_t)(x / e0) * s1;
 }
+// the same in 32 bits: operand blocks are below 2^28 elements (checked by the
+// planner), so element offsets and their byte offsets fit an unsigned
+__device__ __forceinline__ unsigned off2u(int x, int e0, unsigned s0, unsigned s1) {
+  return (unsigned)(x % e0) * s0 + (unsigned)(x / e0) * s1;
+}
 
 constexpr int kThreads = 256;
 
@@ -72,7 +77,7 @@ __device__ __forceinline__ void kwalk_next(KWalk& w, const GemmTask& g, int TK) 
   w.c += TK;
   if (w.c >= g.ek0) {
     w.c = 0;
-    if (++w.k1 >= g.K / g.ek0) {
+    if (++w.k1 >= g.nk1) {
       w.k1 = 0;
       w.t++;
     }
@@ -191,19 +196,35 @@ k_gemm(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cb
       cur_task = tt;
       rsA = block_rsrc(A + g.a_off);
       rsB = block_rsrc(B + g.b_off);
+      // a tile whose rows (columns) all lie in the first folded component
+      // (uniform test) needs no division to place them
+      const unsigned sam0 = (unsigned)g.sam0, sam1 = (unsigned)g.sam1, sak0 = (unsigned)g.sak0;
+      const unsigned sbn0 = (unsigned)g.sbn0, sbn1 = (unsigned)g.sbn1, sbk0 = (unsigned)g.sbk0;
+      if (tl.m0 + TM <= g.em0) {
 #pragma unroll
-      for (int i = 0; i < NA; i++) {
-        const int mg = tl.m0 + ar[i];
-        aoff[i] = mg < cb.M
-                      ? (unsigned)((off2(mg, g.em0, g.sam0, g.sam1) + (int64_t)ak[i] * g.sak0) * ESZ)
-                      : kOOB;
+        for (int i = 0; i < NA; i++) {
+          const int mg = tl.m0 + ar[i];
+          aoff[i] = mg < cb.M ? ((unsigned)mg * sam0 + (unsigned)ak[i] * sak0) * ESZ : kOOB;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+          const int mg = tl.m0 + ar[i];
+          aoff[i] = mg < cb.M ? (off2u(mg, g.em0, sam0, sam1) + (unsigned)ak[i] * sak0) * ESZ : kOOB;
+        }
       }
+      if (tl.n0 + TN <= g.en0) {
 #pragma unroll
-      for (int i = 0; i < NB; i++) {
-        const int ng = tl.n0 + br[i];
-        boff[i] = ng < cb.N
-                      ? (unsigned)((off2(ng, g.en0, g.sbn0, g.sbn1) + (int64_t)bk[i] * g.sbk0) * ESZ)
-                      : kOOB;
+        for (int i = 0; i < NB; i++) {
+          const int ng = tl.n0 + br[i];
+          boff[i] = ng < cb.N ? ((unsigned)ng * sbn0 + (unsigned)bk[i] * sbk0) * ESZ : kOOB;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+          const int ng = tl.n0 + br[i];
+          boff[i] = ng < cb.N ? (off2u(ng, g.en0, sbn0, sbn1) + (unsigned)bk[i] * sbk0) * ESZ : kOOB;
+        }
       }
     };
     auto load = [&](const KWalk& w) {

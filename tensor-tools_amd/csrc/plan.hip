@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(256) k_plan_blocks(MetaView A, MetaView B, Pai
     }
     if (valid && phase == 1) {
       g.kt0 = (int)(kt + incl - ktl);
-      g.pad = 0;
+      g.nk1 = (int)(K / g.ek0);
       ea.tasks[tw + nt + pos] = g;
       if (ea.tuples) {
         int* tu = ea.tuples + 6 * (tw + nt + pos);

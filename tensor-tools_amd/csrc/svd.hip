@@ -1429,6 +1429,7 @@ static GemmTask gtask(int64_t a_off, int64_t b_off, int em0, int64_t sam0, int e
   g.ek0 = ek0; g.sak0 = sak0; g.sak1 = 0; g.sbk0 = sbk0; g.sbk1 = 0;
   g.en0 = en0; g.sbn0 = sbn0; g.sbn1 = 0;
   g.K = K;
+  g.nk1 = K / ek0;
   return g;
 }
 

@@ -13,7 +13,8 @@ struct GemmTask {
   int64_t sam0, sam1, sak0, sak1;
   int64_t sbk0, sbk1, sbn0, sbn1;
   int em0, ek0, en0, K;
-  int kt0, pad;       // first k-tile of this task in its block's k walk
+  int kt0;            // first k-tile of this task in its block's k walk
+  int nk1;            // K / ek0: steps of the second k component
 };
 
 // Output block c: its K-contributions are tasks [t0, t1) (nkt k-tiles in
