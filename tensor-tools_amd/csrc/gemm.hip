@@ -87,6 +87,7 @@ __device__ __forceinline__ void kwalk_next(KWalk& w, const GemmTask& g, int TK) 
 // is <= kt (tasks[t0..t1) hold ascending kt0), then (k1, chunk) inside it
 __device__ __forceinline__ KWalk kwalk_start(const GemmTask* __restrict__ tasks, const CBlock& cb,
                                              int kt, int TK) {
+  if (kt == 0) return KWalk{cb.t0, 0, 0};   // the walk's start (every item without split-K)
   int lo = cb.t0, hi = cb.t1;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
