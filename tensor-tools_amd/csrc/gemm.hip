@@ -511,9 +511,69 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
 // stage of H_eff and of the environment update: K and N are MPO-link sector
 // dims).  One work item = kSkinnyRows rows of one output block, one row per
 // thread; per task the K x N operand block and the K offsets of A go to LDS
-// (zero-padded to kSkinnyK x kSkinnyN), each thread loads its row's K values
-// (coalesced across the wave when A's row stride is 1) and accumulates all
-// N outputs with FMA in T.  A and C are streamed once per task / once.
+// (zero-padded to kSkinnyK x NB, NB = 4, 8 or 16, the smallest that holds the
+// block's N), each thread loads its row's K values (coalesced across the wave
+// when A's row stride is 1) and accumulates its NB outputs with FMA in T.  A
+// and C are streamed once per task / once.
+//
+// One output block's rows for a compile-time column count NB (>= the
+// block's N): the FMAs run over NB columns only.
+template <typename T, typename OutT, int NB>
+__device__ __forceinline__ void skinny_rows(const Tile& tl, const CBlock& cb, const GemmTask* __restrict__ tasks,
+                                            const T* __restrict__ A, const T* __restrict__ B,
+                                            OutT* __restrict__ C, T* Bs, unsigned* koff) {
+  const int m = tl.m0 + (int)threadIdx.x;
+  const bool row_ok = m < cb.M;
+  T acc[NB];
+#pragma unroll
+  for (int n = 0; n < NB; n++) acc[n] = T(0);
+  for (int t = cb.t0; t < cb.t1; t++) {
+    const GemmTask g = tasks[t];
+    __syncthreads();   // previous task's LDS reads are done
+    // the K x N operand block, zero-padded to kSkinnyK x NB; a single folded
+    // component (uniform test) is placed without division
+    const bool k1 = g.K <= g.ek0, n1 = cb.N <= g.en0;
+    for (int e = threadIdx.x; e < kSkinnyK * NB; e += kThreads) {
+      const int k = e / NB, n = e % NB;
+      T v = T(0);
+      if (k < g.K && n < cb.N) {
+        const unsigned ko = k1 ? (unsigned)k * (unsigned)g.sbk0 : off2u(k, g.ek0, (unsigned)g.sbk0, (unsigned)g.sbk1);
+        const unsigned no = n1 ? (unsigned)n * (unsigned)g.sbn0 : off2u(n, g.en0, (unsigned)g.sbn0, (unsigned)g.sbn1);
+        v = B[g.b_off + ko + no];
+      }
+      Bs[e] = v;
+    }
+    if (threadIdx.x < kSkinnyK) {
+      const int k = threadIdx.x;
+      koff[k] = k < g.K ? (k1 ? (unsigned)k * (unsigned)g.sak0 : off2u(k, g.ek0, (unsigned)g.sak0, (unsigned)g.sak1))
+                        : 0u;
+    }
+    __syncthreads();
+    if (!row_ok) continue;
+    const T* ar = A + g.a_off + (m < g.em0 ? (int64_t)m * g.sam0 : off2(m, g.em0, g.sam0, g.sam1));
+    for (int k0 = 0; k0 < g.K; k0 += 8) {
+      T a[8];
+      if (k0 + 8 <= g.K) {   // uniform: no per-element test
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = ar[koff[k0 + u]];
+      } else {
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = k0 + u < g.K ? ar[koff[k0 + u]] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int n = 0; n < NB; n++) acc[n] += a[u] * Bs[(k0 + u) * NB + n];
+    }
+  }
+  if (!row_ok) return;
+  OutT* cr = C + cb.c_off + m;
+#pragma unroll
+  for (int n = 0; n < NB; n++)
+    if (n < cb.N) cr[(int64_t)cb.M * n] = (OutT)acc[n];
+  for (int n = NB; n < cb.N; n++) cr[(int64_t)cb.M * n] = OutT(0);   // blocks without tasks
+}
+
 template <typename T, typename OutT>
 __global__ void __launch_bounds__(kThreads)
 k_skinny(const Tile* __restrict__ tiles, const CBlock* __restrict__ cblk,
@@ -521,46 +581,14 @@ k_skinny(const Tile* __restrict__ tiles, const CBlock* __restrict__ cblk,
          OutT* __restrict__ C) {
   static_assert(kSkinnyRows == kThreads, "one row per thread");
   __shared__ T Bs[kSkinnyK * kSkinnyN];
-  __shared__ int64_t koff[kSkinnyK];
+  __shared__ unsigned koff[kSkinnyK];
   const Tile tl = tiles[blockIdx.x];
   const CBlock cb = cblk[tl.c];
-  const int m = tl.m0 + (int)threadIdx.x;
-  const bool row_ok = m < cb.M;
-  T acc[kSkinnyN];
-#pragma unroll
-  for (int n = 0; n < kSkinnyN; n++) acc[n] = T(0);
-  for (int t = cb.t0; t < cb.t1; t++) {
-    const GemmTask g = tasks[t];
-    __syncthreads();   // previous task's LDS reads are done
-    for (int e = threadIdx.x; e < kSkinnyK * kSkinnyN; e += kThreads) {
-      const int k = e / kSkinnyN, n = e % kSkinnyN;
-      Bs[e] = (k < g.K && n < cb.N) ? B[g.b_off + off2(k, g.ek0, g.sbk0, g.sbk1) +
-                                        off2(n, g.en0, g.sbn0, g.sbn1)]
-                                    : T(0);
-    }
-    if (threadIdx.x < kSkinnyK) {
-      const int k = threadIdx.x;
-      koff[k] = k < g.K ? off2(k, g.ek0, g.sak0, g.sak1) : 0;
-    }
-    __syncthreads();
-    if (!row_ok) continue;
-    const T* ar = A + g.a_off + off2(m, g.em0, g.sam0, g.sam1);
-    for (int k0 = 0; k0 < g.K; k0 += 8) {
-      T a[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) a[u] = k0 + u < g.K ? ar[koff[k0 + u]] : T(0);
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-#pragma unroll
-        for (int n = 0; n < kSkinnyN; n++) acc[n] += a[u] * Bs[(k0 + u) * kSkinnyN + n];
-    }
-  }
-  if (!row_ok) return;
-  OutT* cr = C + cb.c_off + m;
-#pragma unroll
-  for (int n = 0; n < kSkinnyN; n++)
-    if (n < cb.N) cr[(int64_t)cb.M * n] = (OutT)acc[n];
-  for (int n = kSkinnyN; n < cb.N; n++) cr[(int64_t)cb.M * n] = OutT(0);   // blocks without tasks
+  // the column count picks the accumulator width (uniform); blocks without
+  // tasks are written as zeros by every form
+  if (cb.N <= 4) skinny_rows<T, OutT, 4>(tl, cb, tasks, A, B, C, Bs, koff);
+  else if (cb.N <= 8) skinny_rows<T, OutT, 8>(tl, cb, tasks, A, B, C, Bs, koff);
+  else skinny_rows<T, OutT, kSkinnyN>(tl, cb, tasks, A, B, C, Bs, koff);
 }
 
 template <typename T, typename OutT>
