@@ -129,6 +129,10 @@ template <>
 __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
+template <>
+__device__ __forceinline__ unsigned buf_load<unsigned>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
 
 // ------------------------------------------------------------------ FP64/FP32
 // WM x (4/WM) waves; each wave owns (TM/WM) x (TN/WN) of the tile.
@@ -313,18 +317,62 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
-// Operands are FP32 in HBM, split into hi/lo bf16 planes when staged into
-// LDS ([row][k] rows of 32 k, pitch 40 to spread banks), double-buffered with
-// one barrier per k-tile.  A staging slot is one row's 8-element k group;
+// split-BF16 operand word of an FP32 value: bf16 hi (round to nearest even)
+// in the upper half, bf16 of the remainder x - hi in the lower half
+__device__ __forceinline__ unsigned split_word(float x) {
+  __bf16 hi, lo;
+  split_bf16(x, hi, lo);
+  return ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16) | __builtin_bit_cast(unsigned short, lo);
+}
+
+__global__ void k_split_words(const double* __restrict__ s, unsigned* __restrict__ d, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    d[i] = split_word((float)s[i]);
+}
+
+void split_words(const double* src, unsigned* dst, size_t n, hipStream_t s) {
+  const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_split_words, dim3(g), dim3(256), 0, s, src, dst, n);
+}
+
+// eight staged operand values -> their hi and lo bf16 vectors.  FP32 values
+// are split here; pre-split words (split_word) only have their halves
+// regrouped, which gives the same bits.
+__device__ __forceinline__ void split8(const float (&v)[8], v8bf& h, v8bf& l) {
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    __bf16 a, b;
+    split_bf16(v[u], a, b);
+    h[u] = a;
+    l[u] = b;
+  }
+}
+__device__ __forceinline__ void split8(const unsigned (&w)[8], v8bf& h, v8bf& l) {
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  v4u hv, lv;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {   // byte selectors 0-3 pick from the second operand, 4-7 from the first
+    hv[j] = __builtin_amdgcn_perm(w[2 * j + 1], w[2 * j], 0x07060302u);   // hi(w0) | hi(w1) << 16
+    lv[j] = __builtin_amdgcn_perm(w[2 * j + 1], w[2 * j], 0x05040100u);   // lo(w0) | lo(w1) << 16
+  }
+  h = __builtin_bit_cast(v8bf, hv);
+  l = __builtin_bit_cast(v8bf, lv);
+}
+
+// Operands are FP32 in HBM (In = float), split into hi/lo bf16 planes when
+// staged into LDS ([row][k] rows of 32 k, pitch 40 to spread banks),
+// double-buffered with one barrier per k-tile; or pre-split words
+// (In = unsigned, split_word: the same bits, only regrouped when staged).  A staging slot is one row's 8-element k group;
 // per task each slot keeps a 32-bit byte offset into the operand block
 // (buffer loads, out-of-range offsets read zero as in k_gemm), per k-tile
 // the uniform chunk offset and the element's k stride are added.  MFMA
 // operands: lane l holds A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16].
-template <int TM, int TN, int WM, bool AM, bool BK, typename OutT>
+template <int TM, int TN, int WM, bool AM, bool BK, typename OutT, typename In>
 __global__ void __launch_bounds__(kThreads)
 k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cblk,
-              const GemmTask* __restrict__ tasks, const float* __restrict__ A,
-              const float* __restrict__ B, OutT* __restrict__ C, float* __restrict__ part) {
+              const GemmTask* __restrict__ tasks, const In* __restrict__ A,
+              const In* __restrict__ B, OutT* __restrict__ C, float* __restrict__ part) {
   constexpr int TK = 32;
   constexpr int WN = 4 / WM;
   constexpr int FM = TM / (16 * WM), FN = TN / (16 * WN);
@@ -365,7 +413,7 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
     for (int i = 0; i < FM; i++)
 #pragma unroll
       for (int j = 0; j < FN; j++) acc[i][j] = v4f{0, 0, 0, 0};
-    float ra[GA][8], rb[GB][8];
+    In ra[GA][8], rb[GB][8];
     unsigned aoff[GA], boff[GB];
     __amdgpu_buffer_rsrc_t rsA, rsB;
     int cur_task = -1;
@@ -400,25 +448,25 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
 #pragma unroll
         for (int i = 0; i < GA; i++)
 #pragma unroll
-          for (int u = 0; u < 8; u++) ra[i][u] = buf_load<float>(rsA, aoff[i] + (ua + u * sa));
+          for (int u = 0; u < 8; u++) ra[i][u] = buf_load<In>(rsA, aoff[i] + (ua + u * sa));
 #pragma unroll
         for (int i = 0; i < GB; i++)
 #pragma unroll
-          for (int u = 0; u < 8; u++) rb[i][u] = buf_load<float>(rsB, boff[i] + (ub + u * sb));
+          for (int u = 0; u < 8; u++) rb[i][u] = buf_load<In>(rsB, boff[i] + (ub + u * sb));
       } else {
 #pragma unroll
         for (int i = 0; i < GA; i++) {
           const int kl = klim - 8 * akg[i];
 #pragma unroll
           for (int u = 0; u < 8; u++)
-            ra[i][u] = buf_load<float>(rsA, u < kl ? aoff[i] + (ua + u * sa) : kOOB);
+            ra[i][u] = buf_load<In>(rsA, u < kl ? aoff[i] + (ua + u * sa) : kOOB);
         }
 #pragma unroll
         for (int i = 0; i < GB; i++) {
           const int kl = klim - 8 * bkg[i];
 #pragma unroll
           for (int u = 0; u < 8; u++)
-            rb[i][u] = buf_load<float>(rsB, u < kl ? boff[i] + (ub + u * sb) : kOOB);
+            rb[i][u] = buf_load<In>(rsB, u < kl ? boff[i] + (ub + u * sb) : kOOB);
         }
       }
     };
@@ -427,13 +475,7 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
       for (int i = 0; i < GA; i++) {
         if (tid + kThreads * i >= NGA) break;
         v8bf h, l;
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          __bf16 a, b;
-          split_bf16(ra[i][u], a, b);
-          h[u] = a;
-          l[u] = b;
-        }
+        split8(ra[i], h, l);
         *(v8bf*)&AsH[buf][am[i] * KP + akg[i] * 8] = h;
         *(v8bf*)&AsL[buf][am[i] * KP + akg[i] * 8] = l;
       }
@@ -441,13 +483,7 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
       for (int i = 0; i < GB; i++) {
         if (tid + kThreads * i >= NGB) break;
         v8bf h, l;
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          __bf16 a, b;
-          split_bf16(rb[i][u], a, b);
-          h[u] = a;
-          l[u] = b;
-        }
+        split8(rb[i], h, l);
         *(v8bf*)&BsH[buf][bn[i] * KP + bkg[i] * 8] = h;
         *(v8bf*)&BsL[buf][bn[i] * KP + bkg[i] * 8] = l;
       }
@@ -507,6 +543,16 @@ k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restri
 }
 
 // ------------------------------------------------------------------ skinny
+// output element conversion: a plain cast, or a split-BF16 word (OutT = unsigned)
+template <typename OutT, typename T>
+__device__ __forceinline__ OutT to_out(T v) {
+  return (OutT)v;
+}
+template <>
+__device__ __forceinline__ unsigned to_out<unsigned, float>(float v) {
+  return split_word(v);
+}
+
 // Contractions whose tasks all have N <= kSkinnyN and K <= kSkinnyK (the MPO
 // stage of H_eff and of the environment update: K and N are MPO-link sector
 // dims).  One work item = kSkinnyRows rows of one output block, one row per
@@ -570,7 +616,7 @@ __device__ __forceinline__ void skinny_rows(const Tile& tl, const CBlock& cb, co
   OutT* cr = C + cb.c_off + m;
 #pragma unroll
   for (int n = 0; n < NB; n++)
-    if (n < cb.N) cr[(int64_t)cb.M * n] = (OutT)acc[n];
+    if (n < cb.N) cr[(int64_t)cb.M * n] = to_out<OutT>(acc[n]);
   for (int n = NB; n < cb.N; n++) cr[(int64_t)cb.M * n] = OutT(0);   // blocks without tasks
 }
 
@@ -640,12 +686,12 @@ static const void* fp_kernel(bool am, bool bk) {
   if (bk) return (const void*)k_gemm<T, TM, TN, WM, false, true, OutT>;
   return (const void*)k_gemm<T, TM, TN, WM, false, false, OutT>;
 }
-template <int TM, int TN, int WM, typename OutT>
+template <int TM, int TN, int WM, typename OutT, typename In = float>
 static const void* bf_kernel(bool am, bool bk) {
-  if (am && bk) return (const void*)k_gemm_bf16x3<TM, TN, WM, true, true, OutT>;
-  if (am) return (const void*)k_gemm_bf16x3<TM, TN, WM, true, false, OutT>;
-  if (bk) return (const void*)k_gemm_bf16x3<TM, TN, WM, false, true, OutT>;
-  return (const void*)k_gemm_bf16x3<TM, TN, WM, false, false, OutT>;
+  if (am && bk) return (const void*)k_gemm_bf16x3<TM, TN, WM, true, true, OutT, In>;
+  if (am) return (const void*)k_gemm_bf16x3<TM, TN, WM, true, false, OutT, In>;
+  if (bk) return (const void*)k_gemm_bf16x3<TM, TN, WM, false, true, OutT, In>;
+  return (const void*)k_gemm_bf16x3<TM, TN, WM, false, false, OutT, In>;
 }
 template <typename OutT>
 static const void* kernel_of(int prec, int cfg, bool am, bool bk) {
@@ -702,13 +748,13 @@ static void launch_mode(const ttg_plan_s& p, const void* A, const void* B, void*
 #undef TTG_L
 }
 
-template <int TM, int TN, int WM, typename OutT>
+template <int TM, int TN, int WM, typename OutT, typename In>
 static void launch_bf16(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
-  const dim3 grid = persistent_grid(p, bf_kernel<TM, TN, WM, OutT>(p.a_mcontig, p.b_kcontig));
-#define TTG_L(AMv, BKv)                                                                          \
-  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT>), grid, dim3(kThreads), 0, s,   \
-                     p.d_tiles, p.ntiles, p.d_cblk, p.d_tasks, (const float*)A,                \
-                     (const float*)B, (OutT*)C, (float*)p.d_part)
+  const dim3 grid = persistent_grid(p, bf_kernel<TM, TN, WM, OutT, In>(p.a_mcontig, p.b_kcontig));
+#define TTG_L(AMv, BKv)                                                                             \
+  hipLaunchKernelGGL((k_gemm_bf16x3<TM, TN, WM, AMv, BKv, OutT, In>), grid, dim3(kThreads), 0, s, \
+                     p.d_tiles, p.ntiles, p.d_cblk, p.d_tasks, (const In*)A, (const In*)B,       \
+                     (OutT*)C, (float*)p.d_part)
   if (p.a_mcontig && p.b_kcontig) TTG_L(true, true);
   else if (p.a_mcontig) TTG_L(true, false);
   else if (p.b_kcontig) TTG_L(false, true);
@@ -727,11 +773,11 @@ static void by_cfg(const ttg_plan_s& p, const void* A, const void* B, void* C, h
   }
 }
 
-template <typename OutT>
+template <typename OutT, typename In>
 static void by_cfg_bf16(const ttg_plan_s& p, const void* A, const void* B, void* C, hipStream_t s) {
   switch (p.cfg) {
 #define TTG_C(C_, TM, TN, WM) \
-  case C_: launch_bf16<TM, TN, WM, OutT>(p, A, B, C, s); break;
+  case C_: launch_bf16<TM, TN, WM, OutT, In>(p, A, B, C, s); break;
     TTG_TILE_SHAPES(TTG_C)
 #undef TTG_C
     default: break;
@@ -741,8 +787,14 @@ static void by_cfg_bf16(const ttg_plan_s& p, const void* A, const void* B, void*
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s) {
   if (p.ntiles == 0) return 0;
+  TTG_REQUIRE(!(p.packed_in || p.split_out) || p.prec == TTG_BF16X3,
+              "gemm: split-BF16 words are operands of the split-BF16 form only");
   if (p.cfg == TILE_SKINNY) {
-    if (p.prec == TTG_FP64) {
+    TTG_REQUIRE(!p.packed_in, "gemm: the vector-ALU kernel does not read split words");
+    if (p.split_out) {
+      TTG_REQUIRE(!out_f64, "gemm: a split-word output is not FP64");
+      launch_skinny<float, unsigned>(p, A, B, C, s);
+    } else if (p.prec == TTG_FP64) {
       if (out_f64) launch_skinny<double, double>(p, A, B, C, s);
       else launch_skinny<double, float>(p, A, B, C, s);
     } else {
@@ -759,8 +811,14 @@ int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool
     if (out_f64) by_cfg<float, double>(p, A, B, C, s);
     else by_cfg<float, float>(p, A, B, C, s);
   } else {
-    if (out_f64) by_cfg_bf16<double>(p, A, B, C, s);
-    else by_cfg_bf16<float>(p, A, B, C, s);
+    TTG_REQUIRE(!p.split_out, "gemm: the MFMA kernels do not write split words");
+    if (p.packed_in) {
+      if (out_f64) by_cfg_bf16<double, unsigned>(p, A, B, C, s);
+      else by_cfg_bf16<float, unsigned>(p, A, B, C, s);
+    } else {
+      if (out_f64) by_cfg_bf16<double, float>(p, A, B, C, s);
+      else by_cfg_bf16<float, float>(p, A, B, C, s);
+    }
   }
   if (p.prec == TTG_FP64) {
     if (out_f64) launch_reduce<double, double>(p, C, s);

@@ -6,7 +6,11 @@
 // L, W12, R and x does not change inside a Davidson call).  Stage outputs
 // T1 and T2 are materialised in HBM: FP64 for TTG_FP64, FP32 for the
 // reduced-precision forms (whose operands are FP32 shadows of L, W12 and R
-// made once per operator).  The last stage writes y in FP64 directly into
+// made once per operator).  The split-BF16 form keeps the operands of its
+// two MFMA stages (L, x, T2, R) as pre-split words (bf16 hi << 16 | bf16
+// lo, the same 4 bytes per element): L and R are split once per operator,
+// x by the graph's first kernel and T2 by the vector-ALU stage that writes
+// it, so the MFMA stages only regroup halves where they would split.  The last stage writes y in FP64 directly into
 // x's block layout: every output block's offset is remapped to the offset
 // of the same sector tuple in x, so y and x share one block list and the
 // Davidson vector ops stay plain packed-array BLAS-1.
@@ -27,7 +31,10 @@ struct ttg_heff_s {
   ttg_tensor L = nullptr, W = nullptr, R = nullptr;   // not owned (reference: raw pointers)
   void* T1 = nullptr;
   void* T2 = nullptr;
-  float *L32 = nullptr, *W32 = nullptr, *R32 = nullptr, *x32 = nullptr;
+  // reduced-form operand copies: FP32, or split-BF16 words when packed
+  void *Lr = nullptr, *Rr = nullptr, *xr = nullptr;
+  float* W32 = nullptr;
+  bool packed = false;
   double* xbuf = nullptr;        // graph-bound input / output buffers
   double* ybuf = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -38,7 +45,7 @@ struct ttg_heff_s {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     ttg::dfree(T1); ttg::dfree(T2);
-    ttg::dfree(L32); ttg::dfree(W32); ttg::dfree(R32); ttg::dfree(x32);
+    ttg::dfree(Lr); ttg::dfree(W32); ttg::dfree(Rr); ttg::dfree(xr);
     ttg::dfree(xbuf); ttg::dfree(ybuf);
   }
 };
@@ -108,9 +115,10 @@ static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
 // Measured tile-shape choice for an MFMA stage of an operator: each
 // candidate tile shape's work list is built and its launch timed (one
 // untimed, one timed, HIP events on the library stream) on the stage's own
-// operands; the fastest shape is kept.  The choice is cached by a coarse signature of the
-// stage (precision, stage, rounded log2 flops and tasks, operand layout), so
-// the operators of neighbouring DMRG bonds reuse it without timing.  Stages
+// operands; the fastest shape is kept.  The choice is cached by a coarse
+// signature of the stage (precision, stage, rounded log2 flops and tasks,
+// operand layout and format), so the operators of neighbouring DMRG bonds
+// reuse it without timing.  Stages
 // the planner gave the vector-ALU kernel, stages below kTuneMinFlops (their
 // launches are short whatever the shape) and pinned shapes (TTG_TILE_CFG)
 // keep the analytic choice.
@@ -131,7 +139,7 @@ int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, boo
   if (e && *e) return 0;
   const TuneKey key{p->prec, stage, (int)std::lround(std::log2(std::max(p->flops, 1.0))),
                     (int)std::lround(std::log2((double)std::max<int64_t>(p->ntasks, 1))),
-                    (p->a_mcontig ? 1 : 0) | (p->b_kcontig ? 2 : 0)};
+                    (p->a_mcontig ? 1 : 0) | (p->b_kcontig ? 2 : 0) | (p->packed_in ? 4 : 0)};
   auto it = g_tuned.find(key);
   if (it != g_tuned.end()) return it->second == p->cfg ? 0 : rebuild_tiles(*p, it->second);
   hipEvent_t e0, e1;
@@ -162,31 +170,40 @@ int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, boo
   return rebuild_tiles(*p, best);
 }
 
+// operands, output and output type (FP64 or not) of stage st
+struct StageIO {
+  const void* a;
+  const void* b;
+  void* c;
+  bool f64;
+};
+static StageIO stage_io(ttg_heff h, int st) {
+  if (h->prec == TTG_FP64) {
+    if (st == 0) return {h->L->d_data, h->xbuf, h->T1, true};
+    if (st == 1) return {h->T1, h->W->d_data, h->T2, true};
+    return {h->T2, h->R->d_data, h->ybuf, true};
+  }
+  if (st == 0) return {h->Lr, h->xr, h->T1, false};
+  if (st == 1) return {h->T1, h->W32, h->T2, false};
+  return {h->T2, h->Rr, h->ybuf, true};
+}
+
+// n FP64 values into a reduced form's operand format (FP32 or split words)
+static void reduce_copy(const ttg_heff h, const double* src, void* dst, size_t n, hipStream_t s) {
+  if (h->packed) split_words(src, (unsigned*)dst, n, s);
+  else hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(n)), dim3(256), 0, s, src, (float*)dst, n);
+}
+
 static int record(ttg_heff h, hipStream_t s) {
-  const bool red = h->prec != TTG_FP64;
-  const size_t nx = h->xm->total;
-  if (red) {
-    hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(nx)), dim3(256), 0, s, h->xbuf, h->x32, nx);
-    TTG_TRY(gemm_launch(*h->p[0], h->L32, h->x32, h->T1, false, s));
-    TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W32, h->T2, false, s));
-    TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R32, h->ybuf, true, s));
-  } else {
-    TTG_TRY(gemm_launch(*h->p[0], h->L->d_data, h->xbuf, h->T1, true, s));
-    TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W->d_data, h->T2, true, s));
-    TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R->d_data, h->ybuf, true, s));
+  if (h->prec != TTG_FP64) reduce_copy(h, h->xbuf, h->xr, h->xm->total, s);
+  for (int st = 0; st < 3; st++) {
+    const StageIO io = stage_io(h, st);
+    TTG_TRY(gemm_launch(*h->p[st], io.a, io.b, io.c, io.f64, s));
   }
   return 0;
 }
 
-static int to_f32(ttg_tensor t, float** out) {
-  *out = dalloc<float>(t->m->total);
-  TTG_REQUIRE(*out, "heff: allocation failed");
-  hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(t->m->total)), dim3(256), 0, stream(), t->d_data, *out,
-                     t->m->total);
-  return 0;
-}
-
-// intermediates, FP32 operand copies (reduced forms), the measured tile
+// intermediates, reduced-form operand copies (FP32 or split words), the measured tile
 // shapes when `tune`, and the captured apply graph of an operator whose three
 // plans exist; 1 with the error set on failure (the caller deletes h)
 static int heff_finish(ttg_heff h, bool tune) {
@@ -201,22 +218,33 @@ static int heff_finish(ttg_heff h, bool tune) {
     return 1;
   }
   fill_f64(h->ybuf, h->xm->total, 0.0);   // blocks of x never produced stay zero
+  hipStream_t s = stream();
   if (red) {
-    h->x32 = dalloc<float>(h->xm->total);
-    if (!h->x32 || to_f32(h->L, &h->L32) || to_f32(h->W, &h->W32) || to_f32(h->R, &h->R32)) {
+    // split words need the vector-ALU kernel to write T2 (it does for the
+    // MPO stage's narrow sector dimensions) and MFMA kernels to read L, x, T2
+    // and R (small operators can give those stages the vector-ALU kernel,
+    // which reads FP32); otherwise the form stays FP32
+    h->packed = h->prec == TTG_BF16X3 && h->p[1]->cfg == TILE_SKINNY && h->p[0]->cfg != TILE_SKINNY &&
+                h->p[2]->cfg != TILE_SKINNY;
+    h->p[0]->packed_in = h->p[2]->packed_in = h->p[1]->split_out = h->packed;
+    h->xr = dmalloc(std::max<size_t>(1, h->xm->total) * 4);
+    h->Lr = dmalloc(std::max<size_t>(1, h->L->m->total) * 4);
+    h->Rr = dmalloc(std::max<size_t>(1, h->R->m->total) * 4);
+    h->W32 = dalloc<float>(std::max<size_t>(1, h->W->m->total));
+    if (!h->xr || !h->Lr || !h->Rr || !h->W32) {
       set_error("heff: allocation failed");
       return 1;
     }
+    reduce_copy(h, h->L->d_data, h->Lr, h->L->m->total, s);
+    reduce_copy(h, h->R->d_data, h->Rr, h->R->m->total, s);
+    hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(h->W->m->total)), dim3(256), 0, s, h->W->d_data, h->W32,
+                       h->W->m->total);
   }
-  hipStream_t s = stream();
   if (tune) {   // measured tile shapes of the MFMA stages (before the graph is captured)
-    const void* a1 = red ? (const void*)h->L32 : (const void*)h->L->d_data;
-    const void* b1 = red ? (const void*)h->x32 : (const void*)h->xbuf;
-    const void* b2 = red ? (const void*)h->W32 : (const void*)h->W->d_data;
-    const void* b3 = red ? (const void*)h->R32 : (const void*)h->R->d_data;
-    if (tune_stage(h->p[0], 0, a1, b1, h->T1, !red, s) || tune_stage(h->p[1], 1, h->T1, b2, h->T2, !red, s) ||
-        tune_stage(h->p[2], 2, h->T2, b3, h->ybuf, true, s))
-      return 1;
+    for (int st = 0; st < 3; st++) {
+      const StageIO io = stage_io(h, st);
+      if (tune_stage(h->p[st], st, io.a, io.b, io.c, io.f64, s)) return 1;
+    }
   }
   // capture the apply sequence once
   TTG_HIP(hipStreamSynchronize(s));
@@ -364,23 +392,17 @@ int ttg_heff_stage_time(ttg_heff h, ttg_tensor x, int n, double* ms3) {
   hipStream_t s = stream();
   const size_t bytes = h->xm->total * sizeof(double);
   TTG_HIP(hipMemcpyAsync(h->xbuf, x->d_data, bytes, hipMemcpyDeviceToDevice, s));
-  const bool red = h->prec != TTG_FP64;
-  if (red) hipLaunchKernelGGL(k_cvt_f32, dim3(grid_n(h->xm->total)), dim3(256), 0, s, h->xbuf,
-                              h->x32, h->xm->total);
+  if (h->prec != TTG_FP64) reduce_copy(h, h->xbuf, h->xr, h->xm->total, s);
   hipEvent_t ev[4];
   for (auto& e : ev) TTG_HIP(hipEventCreate(&e));
   float acc[3] = {0, 0, 0};
   for (int it = 0; it < n; it++) {
     TTG_HIP(hipEventRecord(ev[0], s));
-    if (red) TTG_TRY(gemm_launch(*h->p[0], h->L32, h->x32, h->T1, false, s));
-    else TTG_TRY(gemm_launch(*h->p[0], h->L->d_data, h->xbuf, h->T1, true, s));
-    TTG_HIP(hipEventRecord(ev[1], s));
-    if (red) TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W32, h->T2, false, s));
-    else TTG_TRY(gemm_launch(*h->p[1], h->T1, h->W->d_data, h->T2, true, s));
-    TTG_HIP(hipEventRecord(ev[2], s));
-    if (red) TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R32, h->ybuf, true, s));
-    else TTG_TRY(gemm_launch(*h->p[2], h->T2, h->R->d_data, h->ybuf, true, s));
-    TTG_HIP(hipEventRecord(ev[3], s));
+    for (int st = 0; st < 3; st++) {
+      const StageIO io = stage_io(h, st);
+      TTG_TRY(gemm_launch(*h->p[st], io.a, io.b, io.c, io.f64, s));
+      TTG_HIP(hipEventRecord(ev[st + 1], s));
+    }
     TTG_HIP(hipEventSynchronize(ev[3]));
     for (int k = 0; k < 3; k++) {
       float t = 0;

@@ -671,7 +671,7 @@ int plan_clone(const ttg_plan_s& src, int prec, ttg_plan* out) {
   p->nblocksC = src.nblocksC;
   p->ntasks = src.ntasks;
   p->flops = src.flops;
-  p->cfg = src.cfg;
+  p->cfg = src.cfg;   // packed_in / split_out start false: the new operator sets them
   p->a_mcontig = src.a_mcontig;
   p->b_kcontig = src.b_kcontig;
   const int64_t nt = std::max<int64_t>(1, p->ntasks);

@@ -91,6 +91,10 @@ struct ttg_plan_s {
   int cfg = ttg::TILE_64x64;
   int ntiles = 0;
   bool a_mcontig = true, b_kcontig = true;
+  // split-BF16 operand format: packed_in, the MFMA stage reads both operands
+  // as pre-split words (bf16 hi << 16 | bf16 lo, see split_word); split_out,
+  // the vector-ALU stage writes its result as such words
+  bool packed_in = false, split_out = false;
   ttg::GemmTask* d_tasks = nullptr;
   ttg::CBlock* d_cblk = nullptr;
   ttg::Tile* d_tiles = nullptr;
@@ -107,10 +111,15 @@ namespace ttg {
 int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
                int force_cfg, ttg_plan* out);
 // launch the grouped GEMM of a plan.  Operand element types follow prec:
-// FP64 reads double A/B; FP32 and BF16X3 read float A/B.  out_f64 selects
-// a double or float C.
+// FP64 reads double A/B; FP32 and BF16X3 read float A/B, or split-BF16
+// words (unsigned) for a BF16X3 plan with packed_in.  out_f64 selects a
+// double or float C; a BF16X3 vector-ALU plan with split_out writes words.
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s);
+// split-BF16 words of n FP64 values: hi = bf16 of the value rounded to FP32,
+// lo = bf16 of the FP32 remainder, word = hi << 16 | lo (the operands the
+// split-BF16 kernel forms from FP32 values when it stages them)
+void split_words(const double* src, unsigned* dst, size_t n, hipStream_t s);
 // workgroups per CU the GEMM kernel of (precision, tile shape, operand
 // layout) can keep resident
 int gemm_occupancy(int prec, int cfg, bool a_mcontig, bool b_kcontig);
