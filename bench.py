@@ -36,15 +36,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def stage_traffic(stage, cfg):
+def stage_traffic(workload, stage, cfg):
     """HBM bytes per launch of one FP64 H_eff stage kernel, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
-    each scaled by what the counter reads for a device copy of known size),
-    or None when no summary for this stage and work shape exists."""
+    rocprofv3 --pmc summary of that workload (FETCH_SIZE and WRITE_SIZE in
+    KiB per dispatch, each scaled by what the counter reads for a device copy
+    of known size), or None when no summary for this workload, stage and work
+    shape exists."""
     path = os.path.join(ROOT, "profiles", "heff_cyl6x12_m2048_traffic.json")
     try:
         d = json.load(open(path))
     except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload:
         return None
     rec = d.get("stages", {}).get(str(stage + 1), {}).get(str(cfg))
     if not rec:
@@ -339,7 +342,7 @@ def main():
     achieved = main_form["stage_flops"][k] / (main_form["stage_ms"][k] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_PEAK_TFLOPS["f64"],
                 "unit": "TFLOP/s", "frac": round(achieved / MFMA_PEAK_TFLOPS["f64"], 4),
-                "traffic": stage_traffic(k, main_form["tile_cfg"][k]),
+                "traffic": stage_traffic(args.workload, k, main_form["tile_cfg"][k]),
                 "kernel": f"stage {k + 1} grouped GEMM (FP64, work shape {main_form['tile_cfg'][k]})"}
 
     cpu = None
