@@ -1005,29 +1005,41 @@ __global__ void __launch_bounds__(kPcThreads) k_pchol_panel(const PcSector* ps, 
   if (tid == 0 && k < kend) rank[blockIdx.x] = k;   // stopped inside this panel
 }
 
-// G[i, j] -= sum_{k in panel rows} R[k, i] R[k, j], 64 x 64 tiles, 256 threads
+// G[i, j] -= sum_{k in panel rows} R[k, i] R[k, j], 64 x 64 tiles, 256 threads.
+// The panel rows' two 64-column pieces are staged in LDS with one batch of
+// loads (a loop over k with a global load per step would wait one memory
+// latency per k); the sums run over k in ascending order.
+constexpr int kPcNbMax = 32;
 __global__ void __launch_bounds__(256) k_pchol_update(const PcSector* ps, int k0, int nb, const int* rank,
                                                       double* A, const double* R) {
+  __shared__ double Ri[kPcNbMax][64];
+  __shared__ double Rj[kPcNbMax][64];
   const PcSector P = ps[blockIdx.y];
   const int n = P.n;
   const int nt = (n + 63) / 64;
   if ((int)blockIdx.x >= nt * nt || k0 >= n) return;
   const int rk = rank[blockIdx.y];
   const int k1 = rk >= 0 ? min(rk, min(k0 + nb, n)) : min(k0 + nb, n);
-  if (k1 <= k0) return;
+  if (k1 <= k0) return;   // uniform over the workgroup
+  const int nk = k1 - k0;   // <= nb <= kPcNbMax (the host caps nb)
   const int i0 = (blockIdx.x % nt) * 64, j0 = (blockIdx.x / nt) * 64;
-  const int i = i0 + (threadIdx.x & 63);
-  const int jb = j0 + (threadIdx.x >> 6) * 16;
+  for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
+    const int kk = e >> 6, c = e & 63;
+    const double* rk_ = R + P.goff + (int64_t)(k0 + kk) * n;
+    Ri[kk][c] = i0 + c < n ? rk_[i0 + c] : 0.0;
+    Rj[kk][c] = j0 + c < n ? rk_[j0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int il = threadIdx.x & 63, jl = (threadIdx.x >> 6) * 16;
+  const int i = i0 + il, jb = j0 + jl;
   if (i >= n) return;
   double acc[16];
 #pragma unroll
   for (int u = 0; u < 16; u++) acc[u] = 0.0;
-  for (int k = k0; k < k1; k++) {
-    const double* rk_ = R + P.goff + (int64_t)k * n;
-    const double ri_ = rk_[i];
+  for (int kk = 0; kk < nk; kk++) {
+    const double ri_ = Ri[kk][il];
 #pragma unroll
-    for (int u = 0; u < 16; u++)
-      if (jb + u < n) acc[u] += ri_ * rk_[jb + u];
+    for (int u = 0; u < 16; u++) acc[u] += ri_ * Rj[kk][jl + u];
   }
 #pragma unroll
   for (int u = 0; u < 16; u++)
@@ -1475,7 +1487,7 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
     nmax = std::max(nmax, nq[q]);
   }
   // panel rows: nb rows of n doubles, d and the flags in the 160 KiB of LDS
-  const int nb = std::max(1, std::min(32, (int)(163840 / (8 * (int64_t)nmax)) - 2));
+  const int nb = std::max(1, std::min(kPcNbMax, (int)(163840 / (8 * (int64_t)nmax)) - 2));
   const size_t lds = sizeof(double) * ((size_t)nb * nmax + nmax) + sizeof(int) * nmax;
   TTG_REQUIRE(lds <= 163840, "svd: sector too large for the pivoted Cholesky panel");
   Scratch sc;
