@@ -533,11 +533,21 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
   if (tid == 0) *sig = 0;
   if (tid < kSub) cnt[tid] = 0;
   {
+    // all of a thread's loads are issued before the first LDS store (a loop
+    // with one load per iteration would wait one memory latency per entry)
+    constexpr int kPer = kSub * kSub / kSolveThreads;
     double* S0 = bj_lds;
     double* Q0 = bj_lds + 2 * kSlotMat;
-    for (int e = tid; e < kSub * kSub; e += kSolveThreads) {
-      const int i = e % kSub, j = e / kSub;   // G is exactly symmetric
-      S0[slot_of(i) * kSP + slot_of(j)] = Gp[S.gp + gidx(i) + (int64_t)S.N * gidx(j)];
+    double gv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = tid + kSolveThreads * u, i = e % kSub, j = e / kSub;   // G is exactly symmetric
+      gv[u] = Gp[S.gp + gidx(i) + (int64_t)S.N * gidx(j)];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = tid + kSolveThreads * u, i = e % kSub, j = e / kSub;
+      S0[slot_of(i) * kSP + slot_of(j)] = gv[u];
       Q0[slot_of(j) * kSP + i] = i == j ? 1.0 : 0.0;
     }
   }
@@ -702,9 +712,16 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
     double* V = Vp + S.gp;
 #pragma unroll
     for (int s = 0; s < 16; s++) af[s] = qb[(4 * s + l4) + kSub * (16 * w + l16)];   // Qb^T
-    for (int e = tid; e < kSub * kSub; e += 256) {   // Bs[k][ii] = V(64a + ii, gj(k))
-      const int ii = e % kSub, kk = e / kSub;
-      Bs[kk * kTLd + ii] = V[(int64_t)(a * kSub + ii) + (int64_t)S.N * gb(kk)];
+    double tv[16];   // Bs[k][ii] = V(64a + ii, gj(k)): all loads issued, then the LDS stores
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = tid + 256 * u, ii = e % kSub, kk = e / kSub;
+      tv[u] = V[(int64_t)(a * kSub + ii) + (int64_t)S.N * gb(kk)];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = tid + 256 * u, ii = e % kSub, kk = e / kSub;
+      Bs[kk * kTLd + ii] = tv[u];
     }
     __syncthreads();
 #pragma unroll
@@ -726,10 +743,20 @@ __global__ void __launch_bounds__(256) k_bjac_apply(const BJSector* sec, const i
   double* G = Gp + S.gp;
 #pragma unroll
   for (int s = 0; s < 16; s++) af[s] = qa[(4 * s + l4) + kSub * (16 * w + l16)];   // Qa^T
-  for (int e = tid; e < kSub * kSub; e += 256) {
-    const int kk = e % kSub, j = e / kSub;
-    Bs[kk * kTLd + j] = G[ga(kk) + (int64_t)S.N * gb(j)];
-    Xs[kk * kTLd + j] = qb[kk + kSub * j];
+  {
+    double tv[16], xv[16];   // all loads issued, then the LDS stores
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = tid + 256 * u, kk = e % kSub, j = e / kSub;
+      tv[u] = G[ga(kk) + (int64_t)S.N * gb(j)];
+      xv[u] = qb[kk + kSub * j];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = tid + 256 * u, kk = e % kSub, j = e / kSub;
+      Bs[kk * kTLd + j] = tv[u];
+      Xs[kk * kTLd + j] = xv[u];
+    }
   }
   __syncthreads();
 #pragma unroll
