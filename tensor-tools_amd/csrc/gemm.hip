@@ -645,7 +645,9 @@ static void launch_skinny(const ttg_plan_s& p, const void* A, const void* B, voi
 }
 
 // ------------------------------------------------------------------ split-K
-// C tile of a split group = sum of its partial slots, added in slot order
+// C tile of a split group = sum of its partial slots, added in slot order.
+// An element's slot values are all loaded before the first add (a loop with
+// one load per slot would wait one memory latency per slot).
 template <typename AccT, typename OutT>
 __global__ void __launch_bounds__(256)
 k_split_reduce(const SplitGroup* __restrict__ groups, const CBlock* __restrict__ cblk,
@@ -657,8 +659,13 @@ k_split_reduce(const SplitGroup* __restrict__ groups, const CBlock* __restrict__
     const int m = gr.m0 + e % TM, n = gr.n0 + e / TM;
     if (m >= cb.M || n >= cb.N) continue;
     const AccT* src = part + (int64_t)gr.slot0 * tile + e;
-    AccT sum = src[0];
-    for (int k = 1; k < gr.ns; k++) sum += src[k * tile];
+    AccT v[kMaxSplit];
+#pragma unroll
+    for (int k = 0; k < kMaxSplit; k++) v[k] = k < gr.ns ? src[k * tile] : AccT(0);
+    AccT sum = v[0];
+#pragma unroll
+    for (int k = 1; k < kMaxSplit; k++)
+      if (k < gr.ns) sum += v[k];
     C[cb.c_off + m + (int64_t)cb.M * n] = (OutT)sum;
   }
 }

@@ -532,7 +532,6 @@ namespace ttg {
 // max(total work over the CUs, the longest item after split-K run at the
 // single-workgroup rate max(mfma, issue, lat)).
 // Split-K (build_tiles) cuts the longest walk into at most kMaxSplit pieces.
-constexpr int kMaxSplit = 16;
 constexpr int kMinPiece = 8;   // fewest k-tiles one split piece owns
 constexpr double kLatency = 2000.0;
 

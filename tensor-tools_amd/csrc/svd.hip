@@ -1032,10 +1032,10 @@ __global__ void __launch_bounds__(kPcThreads) k_pchol_panel(const PcSector* ps, 
   if (tid == 0 && k < kend) rank[blockIdx.x] = k;   // stopped inside this panel
 }
 
-// G[i, j] -= sum_{k in panel rows} R[k, i] R[k, j], 64 x 64 tiles, 256 threads.
-// The panel rows' two 64-column pieces are staged in LDS with one batch of
-// loads (a loop over k with a global load per step would wait one memory
-// latency per k); the sums run over k in ascending order.
+// G[i, j] -= sum_{k in panel rows} R[k, i] R[k, j], 64 x 64 tiles, 256 threads
+// (launched with 256).  The panel rows' two 64-column pieces are staged in LDS
+// with one batch of loads (a loop over k with a global load per step would
+// wait one memory latency per k); the sums run over k in ascending order.
 constexpr int kPcNbMax = 32;
 __global__ void __launch_bounds__(256) k_pchol_update(const PcSector* ps, int k0, int nb, const int* rank,
                                                       double* A, const double* R) {
@@ -1050,11 +1050,22 @@ __global__ void __launch_bounds__(256) k_pchol_update(const PcSector* ps, int k0
   if (k1 <= k0) return;   // uniform over the workgroup
   const int nk = k1 - k0;   // <= nb <= kPcNbMax (the host caps nb)
   const int i0 = (blockIdx.x % nt) * 64, j0 = (blockIdx.x / nt) * 64;
-  for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
-    const int kk = e >> 6, c = e & 63;
-    const double* rk_ = R + P.goff + (int64_t)(k0 + kk) * n;
-    Ri[kk][c] = i0 + c < n ? rk_[i0 + c] : 0.0;
-    Rj[kk][c] = j0 + c < n ? rk_[j0 + c] : 0.0;
+  {
+    constexpr int kPer = kPcNbMax * 64 / 256;   // entries per thread (256 threads)
+    double vi[kPer], vj[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + 256 * u, kk = e >> 6, c = e & 63;
+      const double* rk_ = R + P.goff + (int64_t)(k0 + kk) * n;
+      vi[u] = (kk < nk && i0 + c < n) ? rk_[i0 + c] : 0.0;
+      vj[u] = (kk < nk && j0 + c < n) ? rk_[j0 + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + 256 * u, kk = e >> 6, c = e & 63;
+      Ri[kk][c] = vi[u];
+      Rj[kk][c] = vj[u];
+    }
   }
   __syncthreads();
   const int il = threadIdx.x & 63, jl = (threadIdx.x >> 6) * 16;
@@ -1212,6 +1223,7 @@ __global__ void __launch_bounds__(256) k_svd_rank(int T, int nQ, const int64_t* 
 // summation order: nm over (sector, descending) order, cumsum over the sorted
 // values
 constexpr int kTrChunk = 2048;   // values staged in LDS per pass of the serial loops
+constexpr int kTrPer = kTrChunk / 256;   // staged values per thread (launched with 256 threads)
 // The serial loops add precomputed terms: every thread forms its values'
 // square s*s (for nm) and s*s/nm (for cumsum), each rounded as the
 // reference's expressions round them, and one thread adds them in order, so
@@ -1230,9 +1242,18 @@ __global__ void __launch_bounds__(256) k_svd_truncate(int T, int nQ, const doubl
   double nm = 0.0;
   for (int c0 = 0; c0 < T; c0 += kTrChunk) {
     const int cn = min(kTrChunk, T - c0);
-    for (int e = tid; e < cn; e += blockDim.x) {
-      const double v = svals[c0 + e];
-      term[e] = v * v;
+    {   // a thread's loads are all issued before its stores
+      double v[kTrPer];
+#pragma unroll
+      for (int u = 0; u < kTrPer; u++) {
+        const int e = tid + 256 * u;
+        v[u] = e < cn ? svals[c0 + e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kTrPer; u++) {
+        const int e = tid + 256 * u;
+        if (e < cn) term[e] = v[u] * v[u];
+      }
     }
     __syncthreads();
     if (tid == 0)
@@ -1246,10 +1267,21 @@ __global__ void __launch_bounds__(256) k_svd_truncate(int T, int nQ, const doubl
   for (int c0 = 0; c0 < T; c0 += kTrChunk) {
     if (s_done) break;   // read after a barrier: the same value in every thread
     const int cn = min(kTrChunk, T - c0);
-    for (int e = tid; e < cn; e += blockDim.x) {
-      const double v = sorted[c0 + e];
-      val[e] = v;
-      term[e] = v * v / nm;
+    {
+      double v[kTrPer];
+#pragma unroll
+      for (int u = 0; u < kTrPer; u++) {
+        const int e = tid + 256 * u;
+        v[u] = e < cn ? sorted[c0 + e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kTrPer; u++) {
+        const int e = tid + 256 * u;
+        if (e < cn) {
+          val[e] = v[u];
+          term[e] = v[u] * v[u] / nm;
+        }
+      }
     }
     __syncthreads();
     if (tid == 0) {

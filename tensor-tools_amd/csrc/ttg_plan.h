@@ -19,6 +19,9 @@ struct GemmTask {
 
 // Output block c: its K-contributions are tasks [t0, t1) (nkt k-tiles in
 // total), written once.
+// most pieces one output tile's k walk is split into (split-K)
+constexpr int kMaxSplit = 16;
+
 struct CBlock {
   int64_t c_off;
   int M, N;
