@@ -993,15 +993,21 @@ __global__ void __launch_bounds__(kPcThreads) k_pchol_panel(const PcSector* ps, 
     }
     if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; }
     __syncthreads();
-    if (tid == 0) {
-      double v = rv[0];
-      int i = ri[0];
-      for (int w = 1; w < (int)(blockDim.x >> 6); w++)
-        if (rv[w] > v || (rv[w] == v && ri[w] < i)) { v = rv[w]; i = ri[w]; }
-      s_stop = !(v > tol) || i >= n;
-      s_p = i;
-      s_rkk = s_stop ? 0.0 : sqrt(v);
-      if (!s_stop) piv[i] = 1;
+    if (tid < 64) {   // the waves' candidates, reduced by wave 0 (same order rule)
+      constexpr int kW = kPcThreads / 64;
+      double v = tid < kW ? rv[tid] : -1.0;
+      int i = tid < kW ? ri[tid] : n;
+      for (int o = kW / 2; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o);
+        const int oi = __shfl_xor(i, o);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+      }
+      if (tid == 0) {
+        s_stop = !(v > tol) || i >= n;
+        s_p = i;
+        s_rkk = s_stop ? 0.0 : sqrt(v);
+        if (!s_stop) piv[i] = 1;
+      }
     }
     __syncthreads();
     if (s_stop) break;
@@ -1016,6 +1022,7 @@ __global__ void __launch_bounds__(kPcThreads) k_pchol_panel(const PcSector* ps, 
         v = 0.0;
       } else {
         v = A[P.goff + j + (int64_t)n * p];   // trailing matrix, column p (symmetric)
+#pragma unroll 8
         for (int t = 0; t < ks; t++) v -= Rp[t * n + p] * Rp[t * n + j];
         v *= irkk;
         d[j] -= v * v;
