@@ -593,7 +593,15 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
     double* Sn = bj_lds + ((round & 1) ^ 1) * kSlotMat;
     const double* Qc = bj_lds + (2 + (round & 1)) * kSlotMat;
     double* Qn = bj_lds + (2 + ((round & 1) ^ 1)) * kSlotMat;
+    // lane k (k < 32) of every wave: the rotation of slot k from its 2x2
+    // diagonal block; lanes 32..63 compute slot k - 32 again (same words, no
+    // divergence).  The block's three words are read before the round's S and
+    // Q^T rows (the scheduling barriers keep that order), so the rotation's
+    // arithmetic runs while those 16 reads stream in.
     d2 x0[4], x1[4], q0[4], q1[4], cp[4];
+    const int pd = 2 * (lane & (kBJ - 1));
+    const double app = Sc[pd * kSP + pd], aqq = Sc[(pd + 1) * kSP + pd + 1], apq = Sc[pd * kSP + pd + 1];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       x0[u] = *(const d2*)&Sc[r0 * kSP + 2 * c8 + 16 * u];
@@ -601,13 +609,9 @@ k_bjac_solve(const BJSector* sec, const int2* items, int nitems, int r, const in
       q0[u] = *(const d2*)&Qc[r0 * kSP + 2 * c8 + 16 * u];
       q1[u] = *(const d2*)&Qc[r1 * kSP + 2 * c8 + 16 * u];
     }
-    // lane k (k < 32) of every wave: the rotation of slot k; lanes 32..63
-    // compute slot k - 32 again (same words, no divergence)
+    __builtin_amdgcn_sched_barrier(0);
     double cl, sl;
-    {
-      const int p = 2 * (lane & (kBJ - 1));
-      jac_rotation(Sc[p * kSP + p], Sc[(p + 1) * kSP + p + 1], Sc[p * kSP + p + 1], cl, sl);
-    }
+    jac_rotation(app, aqq, apq, cl, sl);
 #pragma unroll
     for (int u = 0; u < 4; u++) cp[u] = d2{__shfl(cl, c8 + 8 * u), __shfl(sl, c8 + 8 * u)};
     const double c1 = __shfl(cl, kk), s1 = __shfl(sl, kk);
