@@ -633,15 +633,17 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
 // replace a plan's work list by one for tile shape cfg (an MFMA shape)
 int rebuild_tiles(ttg_plan_s& p, int cfg) {
   TTG_REQUIRE(cfg >= 0 && cfg < TILE_NCFG, "plan: bad tile shape");
-  std::vector<CBlock> hcb(p.nblocksC);
-  const D2HPiece pc{hcb.data(), p.d_cblk, sizeof(CBlock) * p.nblocksC};
-  TTG_TRY(d2h_wait(&pc, 1));
+  if ((int)p.hcb.size() != p.nblocksC) {
+    p.hcb.resize(p.nblocksC);
+    const D2HPiece pc{p.hcb.data(), p.d_cblk, sizeof(CBlock) * p.nblocksC};
+    TTG_TRY(d2h_wait(&pc, 1));
+  }
   dfree(p.d_tiles); dfree(p.d_groups); dfree(p.d_part);
   p.d_tiles = nullptr; p.d_groups = nullptr; p.d_part = nullptr;
   p.ngroups = 0;
   p.nslots = 0;
   p.cfg = cfg;
-  return build_tiles(p, hcb);
+  return build_tiles(p, p.hcb);
 }
 
 // k-tile offsets of every task of a block's k walk, and the block's k-tile
@@ -686,10 +688,14 @@ int plan_clone(const ttg_plan_s& src, int prec, ttg_plan* out) {
   if (tk_of(prec) != tk_of(src.prec) && p->nblocksC > 0)
     hipLaunchKernelGGL(k_plan_ktiles, dim3((p->nblocksC + 255) / 256), dim3(256), 0, s, p->nblocksC,
                        p->d_cblk, p->d_tasks, tk_of(prec));
-  std::vector<CBlock> hcb(p->nblocksC);
-  const D2HPiece pc{hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)p->nblocksC};
-  TTG_TRY(d2h_wait(&pc, 1));
-  TTG_TRY(build_tiles(*p, hcb));
+  if (tk_of(prec) == tk_of(src.prec) && (int)src.hcb.size() == p->nblocksC) {
+    p->hcb = src.hcb;   // the same k-tile counts
+  } else {
+    p->hcb.resize(p->nblocksC);
+    const D2HPiece pc{p->hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)p->nblocksC};
+    TTG_TRY(d2h_wait(&pc, 1));
+  }
+  TTG_TRY(build_tiles(*p, p->hcb));
   *out = p.release();
   return 0;
 }
@@ -840,10 +846,10 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   ea.tuples = p->d_tuples;
   hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
                      uM, q, nQ, (int)nblocksC, 1, ba, ea, d_hdr, nA, nB, tk_of(prec));
-  std::vector<CBlock> hcb(nblocksC);
-  const D2HPiece pc{hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)nblocksC};
+  p->hcb.resize(nblocksC);
+  const D2HPiece pc{p->hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)nblocksC};
   TTG_TRY(d2h_wait(&pc, 1));
-  TTG_TRY(build_tiles(*p, hcb));
+  TTG_TRY(build_tiles(*p, p->hcb));
   *out = p.release();
   return 0;
 }

@@ -107,6 +107,10 @@ struct ttg_plan_s {
   int64_t nslots = 0;
   // 6-tuples (c, a, b, M, N, K) for the bookkeeping checks
   int* d_tuples = nullptr;
+  // host copy of d_cblk as planned (the fields a work list needs: M, N and
+  // k-tile counts; c_off is the planned one, before any remap on the device),
+  // so a work list can be rebuilt without a read-back
+  std::vector<ttg::CBlock> hcb;
   ~ttg_plan_s();
 };
 
