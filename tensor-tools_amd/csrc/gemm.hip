@@ -793,6 +793,7 @@ static void by_cfg_bf16(const ttg_plan_s& p, const void* A, const void* B, void*
 
 int gemm_launch(const ttg_plan_s& p, const void* A, const void* B, void* C, bool out_f64,
                 hipStream_t s) {
+  TTG_REQUIRE(!p.tiles_pending, "gemm: the plan's work list is not built");
   if (p.ntiles == 0) return 0;
   TTG_REQUIRE(!(p.packed_in || p.split_out) || p.prec == TTG_BF16X3,
               "gemm: split-BF16 words are operands of the split-BF16 form only");

@@ -134,14 +134,16 @@ constexpr double kTuneMinFlops = 2e8;
 int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, bool out_f64,
                hipStream_t s) {
   PhaseTimer _phase("tune_stage");
-  if (p->cfg == TILE_SKINNY || p->ntasks == 0 || p->flops < kTuneMinFlops) return 0;
+  // a plan built with its work list deferred gets it here, for the shape kept
+  auto keep = [&]() { return p->tiles_pending ? rebuild_tiles(*p, p->cfg) : 0; };
+  if (p->cfg == TILE_SKINNY || p->ntasks == 0 || p->flops < kTuneMinFlops) return keep();
   const char* e = getenv("TTG_TILE_CFG");
-  if (e && *e) return 0;
+  if (e && *e) return keep();
   const TuneKey key{p->prec, stage, (int)std::lround(std::log2(std::max(p->flops, 1.0))),
                     (int)std::lround(std::log2((double)std::max<int64_t>(p->ntasks, 1))),
                     (p->a_mcontig ? 1 : 0) | (p->b_kcontig ? 2 : 0) | (p->packed_in ? 4 : 0)};
   auto it = g_tuned.find(key);
-  if (it != g_tuned.end()) return it->second == p->cfg ? 0 : rebuild_tiles(*p, it->second);
+  if (it != g_tuned.end()) return it->second == p->cfg ? keep() : rebuild_tiles(*p, it->second);
   hipEvent_t e0, e1;
   TTG_HIP(hipEventCreate(&e0));
   TTG_HIP(hipEventCreate(&e1));
@@ -288,7 +290,8 @@ int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_lik
     std::vector<int> pa(np), pb(np);
     for (int i = 0; i < np; i++) { pa[i] = *pp++; pb[i] = *pp++; }
     MetaPtr a = s == 0 ? lhs[0] : h->p[s - 1]->C;
-    int r = plan_build(a, rhs[s], np, pa.data(), pb.data(), prec, -1, &h->p[s]);
+    // work lists deferred: heff_finish builds each for its measured shape
+    int r = plan_build(a, rhs[s], np, pa.data(), pb.data(), prec, -1, &h->p[s], true);
     if (r) { delete h; return r; }
     h->flops += h->p[s]->flops;
   }

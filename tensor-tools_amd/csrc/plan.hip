@@ -632,7 +632,7 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
 
 // replace a plan's work list by one for tile shape cfg (an MFMA shape)
 int rebuild_tiles(ttg_plan_s& p, int cfg) {
-  TTG_REQUIRE(cfg >= 0 && cfg < TILE_NCFG, "plan: bad tile shape");
+  TTG_REQUIRE(cfg >= 0 && (cfg < TILE_NCFG || cfg == TILE_SKINNY), "plan: bad tile shape");
   if ((int)p.hcb.size() != p.nblocksC) {
     p.hcb.resize(p.nblocksC);
     const D2HPiece pc{p.hcb.data(), p.d_cblk, sizeof(CBlock) * p.nblocksC};
@@ -643,6 +643,7 @@ int rebuild_tiles(ttg_plan_s& p, int cfg) {
   p.ngroups = 0;
   p.nslots = 0;
   p.cfg = cfg;
+  p.tiles_pending = false;
   return build_tiles(p, p.hcb);
 }
 
@@ -712,7 +713,7 @@ static PlanHdr* pinned_hdr() {
 }
 
 int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
-               int force_cfg, ttg_plan* out) {
+               int force_cfg, ttg_plan* out, bool defer_tiles) {
   PhaseTimer _phase("plan_build");
   TTG_TRY(ensure_init());
   TTG_REQUIRE(np >= 0 && np <= A->rank && np <= B->rank, "plan: bad pair count");
@@ -849,7 +850,8 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   p->hcb.resize(nblocksC);
   const D2HPiece pc{p->hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)nblocksC};
   TTG_TRY(d2h_wait(&pc, 1));
-  TTG_TRY(build_tiles(*p, p->hcb));
+  if (defer_tiles) p->tiles_pending = true;
+  else TTG_TRY(build_tiles(*p, p->hcb));
   *out = p.release();
   return 0;
 }

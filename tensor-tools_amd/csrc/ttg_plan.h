@@ -111,12 +111,17 @@ struct ttg_plan_s {
   // k-tile counts; c_off is the planned one, before any remap on the device),
   // so a work list can be rebuilt without a read-back
   std::vector<ttg::CBlock> hcb;
+  // the work list is not built yet (plan_build with defer_tiles: the caller
+  // chooses the tile shape first, tune_stage)
+  bool tiles_pending = false;
   ~ttg_plan_s();
 };
 
 namespace ttg {
+// defer_tiles: leave the work list unbuilt (tiles_pending) for a caller that
+// picks the tile shape itself before the first launch (tune_stage)
 int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const int* pb, int prec,
-               int force_cfg, ttg_plan* out);
+               int force_cfg, ttg_plan* out, bool defer_tiles = false);
 // launch the grouped GEMM of a plan.  Operand element types follow prec:
 // FP64 reads double A/B; FP32 and BF16X3 read float A/B, or split-BF16
 // words (unsigned) for a BF16X3 plan with packed_in.  out_f64 selects a
