@@ -1573,15 +1573,34 @@ static int lr_eigen(const std::vector<int>& big, const std::vector<int>& nq, con
   TTG_HIP(hipFuncSetAttribute((const void*)k_pchol_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   const int nt = (nmax + 63) / 64;
-  for (int k0 = 0; k0 < nmax; k0 += nb) {
-    hipLaunchKernelGGL(k_pchol_panel, dim3(ns), dim3(kPcThreads), lds, s, d_ps, k0, nb, G, R, dg, pg, rank,
-                       dmax);
-    hipLaunchKernelGGL(k_pchol_update, dim3(nt * nt, ns), dim3(256), 0, s, d_ps, k0, nb, rank, G, R);
-  }
+  // Panels are launched up to one more than the previous call needed, then
+  // the ranks are read back; only if a sector is still running are the rest
+  // launched (a finished sector's launches exit at once, but each launch
+  // still costs a few microseconds).  The panels run in the same order
+  // either way.
+  static int prev_panels = 0;
+  const int all_panels = (nmax + nb - 1) / nb;
+  int target = prev_panels > 0 ? std::min(all_panels, prev_panels + 1) : all_panels;
   std::vector<int> rk(ns);
-  {
+  int k0 = 0;
+  for (;;) {
+    for (; k0 < target * nb && k0 < nmax; k0 += nb) {
+      hipLaunchKernelGGL(k_pchol_panel, dim3(ns), dim3(kPcThreads), lds, s, d_ps, k0, nb, G, R, dg, pg, rank,
+                         dmax);
+      hipLaunchKernelGGL(k_pchol_update, dim3(nt * nt, ns), dim3(256), 0, s, d_ps, k0, nb, rank, G, R);
+    }
     const D2HPiece pc{rk.data(), rank, sizeof(int) * ns};
     TTG_TRY(d2h_wait(&pc, 1));
+    bool running = false;   // a sector with neither a rank nor all its columns done
+    for (int li = 0; li < ns; li++) running |= rk[li] < 0 && k0 < ps[li].n;
+    if (!running || k0 >= nmax) break;
+    target = all_panels;
+  }
+  {
+    int need = 0;   // panels this call needed: up to each sector's stopping panel
+    for (int li = 0; li < ns; li++)
+      need = std::max(need, rk[li] < 0 ? (ps[li].n + nb - 1) / nb : rk[li] / nb + 1);
+    prev_panels = std::min(need, all_panels);
   }
   std::vector<int> r2(ns);
   std::vector<int64_t> goff2(ns + 1, 0), poff2(ns + 1, 0), eoff2(ns + 1, 0);
