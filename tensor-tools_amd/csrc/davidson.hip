@@ -89,6 +89,7 @@ __global__ void k_embed_copy(int nb, const int* map, const int64_t* soff, const 
     const int db = map[b];
     if (db < 0) continue;
     const int64_t n = soff[b + 1] - soff[b];
+#pragma unroll 8
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) d[doff[db] + i] = s[soff[b] + i];
   }
 }
@@ -102,6 +103,7 @@ __global__ void __launch_bounds__(256) k_dot_mapped(int nb, const int* map, cons
     const int yb = map[b];
     if (yb < 0) continue;
     const int64_t n = xoff[b + 1] - xoff[b];
+#pragma unroll 8
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += x[xoff[b] + i] * y[yoff[yb] + i];
   }
   __shared__ double red[256];

@@ -188,6 +188,7 @@ __global__ void k_svd_gather(MetaView A, int nleft, const double* __restrict__ a
     double* dst = mat + moff[q] + t.loff[il] + rows * t.roff[ir];
     const double* src = a + A.off[b];
     const int64_t n = ls * rs;
+#pragma unroll 8
     for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
       const int64_t i = e % ls, j = e / ls;
       dst[i + rows * j] = src[e];
@@ -274,6 +275,7 @@ k_jacobi(const int* __restrict__ nq, const int64_t* __restrict__ goff,
   int* tab = (lay.where & 4) ? (int*)next : nullptr;
   const double* G0 = Gin + goff[q];    // full n x n column-major
   // load packed triangle (pad row/col zero), V = I, block-pair table
+#pragma unroll 8
   for (int64_t e = threadIdx.x; e < psize; e += blockDim.x) {
     int i, j;
     tri_index(e, i, j);
@@ -1149,6 +1151,7 @@ __global__ void __launch_bounds__(256) k_svd_select(const int* nq, const int* kq
   for (int c0 = 0; c0 < n; c0 += kRankChunk) {
     const int cn = min(kRankChunk, n - c0);
     __syncthreads();
+#pragma unroll 8
     for (int e = threadIdx.x; e < cn; e += 256) chunk[e] = rank_key(ev[c0 + e]);
     __syncthreads();
     for (int j = 0; j < cn; j++) {
@@ -1267,8 +1270,11 @@ __global__ void __launch_bounds__(256) k_svd_truncate(int T, int nQ, const doubl
       }
     }
     __syncthreads();
-    if (tid == 0)
+    if (tid == 0) {
+      // unrolled so that the LDS reads run ahead of the (ordered) adds
+#pragma unroll 8
       for (int e = 0; e < cn; e++) nm += term[e];
+    }
     __syncthreads();
   }
   if (tid == 0) { s_nm = nm; s_cum = 0.0; s_acc = 0; s_done = 0; }
@@ -1299,6 +1305,7 @@ __global__ void __launch_bounds__(256) k_svd_truncate(int T, int nQ, const doubl
       double cumsum = s_cum;
       long long acc = s_acc;
       int done = 0;
+#pragma unroll 8
       for (int e = 0; e < cn; e++) {
         cumsum += term[e];
         if (val[e] == 0. && cutoff != 0.) { done = 1; break; }
