@@ -165,14 +165,20 @@ __global__ void __launch_bounds__(1024) k_plan_unique(KeyArrays k, int nA, int n
   }
   __syncthreads();
   carry = 0;
+  int64_t ncand = 0;   // every result block of group t tries each of its nM mids
   for (int base = 0; base < nQ; base += 1024) {
     int t = base + threadIdx.x;
-    int64_t cnt = 0;
-    if (t < nQ) cnt = (int64_t)(q.qL1[t] - q.qL0[t]) * (q.qR1[t] - q.qR0[t]);
-    int64_t tot;
+    int64_t cnt = 0, cand = 0;
+    if (t < nQ) {
+      cnt = (int64_t)(q.qL1[t] - q.qL0[t]) * (q.qR1[t] - q.qR0[t]);
+      cand = cnt * (q.qM1[t] - q.qM0[t]);
+    }
+    int64_t tot, tot2;
     int64_t pos = block_excl_scan<1024>(cnt, lds, &tot);
+    (void)block_excl_scan<1024>(cand, lds, &tot2);
     if (t < nQ) q.qbase[t] = carry + pos;
     carry += tot;
+    ncand += tot2;
   }
   if (threadIdx.x == 0) {
     q.qbase[nQ] = carry;
@@ -181,6 +187,7 @@ __global__ void __launch_bounds__(1024) k_plan_unique(KeyArrays k, int nA, int n
     hdr->nuM = nuM;
     hdr->nQ = nQ;
     hdr->nblocksC = carry;
+    hdr->ncand = ncand;
   }
 }
 
@@ -783,7 +790,9 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   TTG_HIP(hipStreamSynchronize(s));
   const int64_t nblocksC = h_hdr->nblocksC;
   const int nQ = (int)h_hdr->nQ;
+  const int64_t ncand = h_hdr->ncand;
   TTG_REQUIRE(nblocksC < (1 << 30), "plan: too many result blocks");
+  TTG_REQUIRE(ncand < (1 << 30), "plan: too many block pairs");
 
   // result structure
   std::vector<std::pair<const TMeta*, int>> cidx;
@@ -801,18 +810,41 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   ba.c_qi = C->d_qi;
   EmitArrays ea{};
   const unsigned gb = (unsigned)std::max<int64_t>(1, (nblocksC + 3) / 4);   // one wave per result block
+  // the task list is emitted right after the count pass, into arrays sized
+  // by the candidate count (an upper bound of the task count), so one
+  // read-back (header and block table) serves both passes
+  std::unique_ptr<ttg_plan_s> p(new ttg_plan_s());
+  const int64_t ncap = std::max<int64_t>(1, ncand);
+  p->d_tasks = dalloc<GemmTask>(ncap);
+  p->d_cblk = dalloc<CBlock>(std::max<int64_t>(1, nblocksC));
+  p->d_tuples = dalloc<int>(6 * ncap);
+  if (!p->d_tasks || !p->d_cblk || !p->d_tuples) {
+    set_error(std::string("plan: ") + ttg_last_error());
+    return 1;
+  }
   hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
                      uM, q, nQ, (int)nblocksC, 0, ba, ea, d_hdr, nA, nB, tk_of(prec));
   hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, s, (int)nblocksC, ba.size, ba.ntask,
                      ba.flops, C->d_off, task_off, d_hdr);
-  TTG_HIP(hipMemcpyAsync(h_hdr, d_hdr, sizeof(PlanHdr), hipMemcpyDeviceToHost, s));
-  TTG_HIP(hipStreamSynchronize(s));
+  ba.task_off = task_off;
+  ba.c_off = C->d_off;
+  ea.tasks = p->d_tasks;
+  ea.cblk = p->d_cblk;
+  ea.tuples = p->d_tuples;
+  hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
+                     uM, q, nQ, (int)nblocksC, 1, ba, ea, d_hdr, nA, nB, tk_of(prec));
+  p->hcb.resize(nblocksC);
+  {
+    const D2HPiece pc[2] = {{h_hdr, d_hdr, sizeof(PlanHdr)},
+                            {p->hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)nblocksC}};
+    TTG_TRY(d2h_wait(pc, 2));
+  }
   TTG_REQUIRE(!h_hdr->err, "plan: operand layout needs more than two strides per GEMM dimension");
+  TTG_REQUIRE(h_hdr->ntasks <= ncand, "plan: task count above its bound");
   C->total = (size_t)h_hdr->totalC;
   // the GEMM's 32-bit byte offsets per block
   TTG_REQUIRE(h_hdr->opmax < (int64_t(1) << 28), "plan: an operand block exceeds 2^28 elements");
 
-  std::unique_ptr<ttg_plan_s> p(new ttg_plan_s());
   p->A = A;
   p->B = B;
   p->C = C;
@@ -821,7 +853,6 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   p->nblocksC = (int)nblocksC;
   p->ntasks = h_hdr->ntasks;
   p->flops = h_hdr->flops;
-  const int64_t nt = std::max<int64_t>(1, p->ntasks);
   p->a_mcontig = h_hdr->votes[0] >= h_hdr->votes[1];
   p->b_kcontig = h_hdr->votes[2] >= h_hdr->votes[3];
   if (force_cfg < 0) {   // TTG_TILE_CFG=<n> pins every plan's tile shape (tuning runs)
@@ -833,23 +864,6 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   else if (force_cfg >= 0 && force_cfg < TILE_NCFG) p->cfg = force_cfg;
   else p->cfg = skinny_ok ? TILE_SKINNY
                           : choose_cfg(*h_hdr, prec, rt().num_cu, p->a_mcontig, p->b_kcontig);
-  p->d_tasks = dalloc<GemmTask>(nt);
-  p->d_cblk = dalloc<CBlock>(std::max<int64_t>(1, nblocksC));
-  p->d_tuples = dalloc<int>(6 * nt);
-  if (!p->d_tasks || !p->d_cblk || !p->d_tuples) {
-    set_error(std::string("plan: ") + ttg_last_error());
-    return 1;
-  }
-  ba.task_off = task_off;
-  ba.c_off = C->d_off;
-  ea.tasks = p->d_tasks;
-  ea.cblk = p->d_cblk;
-  ea.tuples = p->d_tuples;
-  hipLaunchKernelGGL(k_plan_blocks, dim3(gb), dim3(256), 0, s, A->view(), B->view(), ps, k, uL, uR,
-                     uM, q, nQ, (int)nblocksC, 1, ba, ea, d_hdr, nA, nB, tk_of(prec));
-  p->hcb.resize(nblocksC);
-  const D2HPiece pc{p->hcb.data(), p->d_cblk, sizeof(CBlock) * (size_t)nblocksC};
-  TTG_TRY(d2h_wait(&pc, 1));
   if (defer_tiles) p->tiles_pending = true;
   else TTG_TRY(build_tiles(*p, p->hcb));
   *out = p.release();

@@ -80,6 +80,7 @@ struct PlanHdr {
   int64_t nmax, ktmax;           // widest output block with tasks; largest task K
   int64_t opmax;                 // largest operand block (elements) of any task
   int64_t err;
+  int64_t ncand;                 // (result block, mid) candidates: an upper bound of ntasks
 };
 
 }  // namespace ttg
