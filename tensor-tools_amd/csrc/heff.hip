@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <map>
+#include <set>
 #include <tuple>
 
 #include "ttg_device.h"
@@ -144,9 +145,6 @@ int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, boo
                     (p->a_mcontig ? 1 : 0) | (p->b_kcontig ? 2 : 0) | (p->packed_in ? 4 : 0)};
   auto it = g_tuned.find(key);
   if (it != g_tuned.end()) return it->second == p->cfg ? keep() : rebuild_tiles(*p, it->second);
-  hipEvent_t e0, e1;
-  TTG_HIP(hipEventCreate(&e0));
-  TTG_HIP(hipEventCreate(&e1));
   int best = p->cfg;
   float best_ms = 1e30f;
   // candidates: the analytic choice and the shapes that win on the DMRG
@@ -155,19 +153,49 @@ int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, boo
   std::vector<int> cand{p->cfg};
   for (int c : {TILE_32x32, TILE_64x32, TILE_32x64, TILE_96x32, TILE_32x96, TILE_96x96})
     if (c != p->cfg) cand.push_back(c);
-  for (int c : cand) {
-    if (rebuild_tiles(*p, c)) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return 1; }
-    TTG_TRY(gemm_launch(*p, A, B, C, out_f64, s));
-    TTG_HIP(hipEventRecord(e0, s));
-    TTG_TRY(gemm_launch(*p, A, B, C, out_f64, s));
-    TTG_HIP(hipEventRecord(e1, s));
-    TTG_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    TTG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) { best_ms = ms; best = c; }
+  // One timed launch per candidate, all queued before one wait.  A kernel
+  // instantiation's first launch in the process (code object load) is run
+  // untimed first.  Work lists are rebuilt between candidates in stream
+  // order (their uploads queue behind the launches that read the old ones).
+  static std::set<int> warmed;
+  std::vector<hipEvent_t> ev(2 * cand.size(), nullptr);
+  auto release = [&]() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  for (size_t i = 0; i < cand.size(); i++) {
+    if (hipEventCreate(&ev[2 * i]) != hipSuccess || hipEventCreate(&ev[2 * i + 1]) != hipSuccess) {
+      release();
+      set_error("heff: event creation failed");
+      return 1;
+    }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  for (size_t i = 0; i < cand.size(); i++) {
+    const int c = cand[i];
+    const int kid = c | (p->prec << 4) | (p->a_mcontig ? 64 : 0) | (p->b_kcontig ? 128 : 0) |
+                    (out_f64 ? 256 : 0) | (p->packed_in ? 512 : 0);
+    if (rebuild_tiles(*p, c) || (!warmed.count(kid) && gemm_launch(*p, A, B, C, out_f64, s)) ||
+        hipEventRecord(ev[2 * i], s) != hipSuccess || gemm_launch(*p, A, B, C, out_f64, s) ||
+        hipEventRecord(ev[2 * i + 1], s) != hipSuccess) {
+      release();
+      set_error(std::string("heff: tile timing failed: ") + ttg_last_error());
+      return 1;
+    }
+    warmed.insert(kid);
+  }
+  if (hipEventSynchronize(ev.back()) != hipSuccess) {
+    release();
+    set_error("heff: tile timing failed");
+    return 1;
+  }
+  for (size_t i = 0; i < cand.size(); i++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess && ms < best_ms) {
+      best_ms = ms;
+      best = cand[i];
+    }
+  }
+  release();
   g_tuned[key] = best;
   return rebuild_tiles(*p, best);
 }
