@@ -10,10 +10,11 @@
 // two MFMA stages (L, x, T2, R) as pre-split words (bf16 hi << 16 | bf16
 // lo, the same 4 bytes per element): L and R are split once per operator,
 // x by the graph's first kernel and T2 by the vector-ALU stage that writes
-// it, so the MFMA stages only regroup halves where they would split.  The last stage writes y in FP64 directly into
-// x's block layout: every output block's offset is remapped to the offset
-// of the same sector tuple in x, so y and x share one block list and the
-// Davidson vector ops stay plain packed-array BLAS-1.
+// it, so the MFMA stages only regroup halves where they would split.  The
+// last stage writes y in FP64 directly into x's block layout: every output
+// block's offset is remapped to the offset of the same sector tuple in x, so
+// y and x share one block list and the Davidson vector ops stay plain
+// packed-array BLAS-1.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -114,15 +115,15 @@ static int remap_to_x(ttg_plan p3, const MetaPtr& xm) {
 }
 
 // Measured tile-shape choice for an MFMA stage of an operator: each
-// candidate tile shape's work list is built and its launch timed (one
-// untimed, one timed, HIP events on the library stream) on the stage's own
-// operands; the fastest shape is kept.  The choice is cached by a coarse
-// signature of the stage (precision, stage, rounded log2 flops and tasks,
-// operand layout and format), so the operators of neighbouring DMRG bonds
-// reuse it without timing.  Stages
-// the planner gave the vector-ALU kernel, stages below kTuneMinFlops (their
+// candidate tile shape's work list is built and its launch timed (HIP events
+// on the library stream) on the stage's own operands; the fastest shape is
+// kept.  The choice is cached by a coarse signature of the stage (precision,
+// stage, rounded log2 flops and tasks, operand layout and format), so the
+// operators of neighbouring DMRG bonds reuse it without timing.  Stages the
+// planner gave the vector-ALU kernel, stages below kTuneMinFlops (their
 // launches are short whatever the shape) and pinned shapes (TTG_TILE_CFG)
-// keep the analytic choice.
+// keep the analytic choice.  A plan built with its work list deferred gets
+// the list here, for the shape kept.
 struct TuneKey {
   int prec, stage, lfl, lnt, layout;
   bool operator<(const TuneKey& o) const {
@@ -135,7 +136,6 @@ constexpr double kTuneMinFlops = 2e8;
 int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, bool out_f64,
                hipStream_t s) {
   PhaseTimer _phase("tune_stage");
-  // a plan built with its work list deferred gets it here, for the shape kept
   auto keep = [&]() { return p->tiles_pending ? rebuild_tiles(*p, p->cfg) : 0; };
   if (p->cfg == TILE_SKINNY || p->ntasks == 0 || p->flops < kTuneMinFlops) return keep();
   const char* e = getenv("TTG_TILE_CFG");
