@@ -363,11 +363,12 @@ __device__ __forceinline__ void split8(const unsigned (&w)[8], v8bf& h, v8bf& l)
 // Operands are FP32 in HBM (In = float), split into hi/lo bf16 planes when
 // staged into LDS ([row][k] rows of 32 k, pitch 40 to spread banks),
 // double-buffered with one barrier per k-tile; or pre-split words
-// (In = unsigned, split_word: the same bits, only regrouped when staged).  A staging slot is one row's 8-element k group;
-// per task each slot keeps a 32-bit byte offset into the operand block
-// (buffer loads, out-of-range offsets read zero as in k_gemm), per k-tile
-// the uniform chunk offset and the element's k stride are added.  MFMA
-// operands: lane l holds A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16].
+// (In = unsigned, split_word: the same bits, only regrouped when staged).
+// A staging slot is one row's 8-element k group; per task each slot keeps a
+// 32-bit byte offset into the operand block (buffer loads, out-of-range
+// offsets read zero as in k_gemm), per k-tile the uniform chunk offset and
+// the element's k stride are added.  MFMA operands: lane l holds
+// A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16].
 template <int TM, int TN, int WM, bool AM, bool BK, typename OutT, typename In>
 __global__ void __launch_bounds__(kThreads)
 k_gemm_bf16x3(const Tile* __restrict__ tiles, int ntiles, const CBlock* __restrict__ cblk,
