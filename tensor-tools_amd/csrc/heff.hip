@@ -155,8 +155,9 @@ int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, boo
     if (c != p->cfg) cand.push_back(c);
   // One timed launch per candidate, all queued before one wait.  A kernel
   // instantiation's first launch in the process (code object load) is run
-  // untimed first.  Work lists are rebuilt between candidates in stream
-  // order (their uploads queue behind the launches that read the old ones).
+  // untimed first, and the first candidate twice.  Work lists are rebuilt
+  // between candidates in stream order (their uploads queue behind the
+  // launches that read the old ones).
   static std::set<int> warmed;
   std::vector<hipEvent_t> ev(2 * cand.size(), nullptr);
   auto release = [&]() {
@@ -174,7 +175,12 @@ int tune_stage(ttg_plan p, int stage, const void* A, const void* B, void* C, boo
     const int c = cand[i];
     const int kid = c | (p->prec << 4) | (p->a_mcontig ? 64 : 0) | (p->b_kcontig ? 128 : 0) |
                     (out_f64 ? 256 : 0) | (p->packed_in ? 512 : 0);
-    if (rebuild_tiles(*p, c) || (!warmed.count(kid) && gemm_launch(*p, A, B, C, out_f64, s)) ||
+    // (the first candidate's extra launch: it is not timed on an idle GPU's
+    // lower clock)
+    const int warm = i == 0 ? 2 : (warmed.count(kid) ? 0 : 1);
+    int wr = rebuild_tiles(*p, c);
+    for (int k = 0; k < warm && !wr; k++) wr = gemm_launch(*p, A, B, C, out_f64, s);
+    if (wr ||
         hipEventRecord(ev[2 * i], s) != hipSuccess || gemm_launch(*p, A, B, C, out_f64, s) ||
         hipEventRecord(ev[2 * i + 1], s) != hipSuccess) {
       release();
