@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -108,10 +109,26 @@ class Scratch {
   ~Scratch() {
     for (void* p : bufs_) dfree(p);
   }
+  // Buffers up to kSmall bytes are carved (256-byte aligned) from shared
+  // chunks of kChunk bytes, so a planning call's two dozen small tables cost
+  // one or two pool allocations instead of one each; larger ones are pool
+  // allocations of their own.
   template <class T>
   T* get(size_t n) {
-    T* p = dalloc<T>(n);
-    hold(p);
+    const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~size_t(255);
+    if (bytes > kSmall) {
+      T* p = dalloc<T>(n);
+      hold(p);
+      return p;
+    }
+    if (!chunk_ || used_ + bytes > kChunk) {
+      chunk_ = (char*)dmalloc(kChunk);
+      hold(chunk_);
+      used_ = 0;
+      if (!chunk_) return nullptr;
+    }
+    T* p = (T*)(chunk_ + used_);
+    used_ += bytes;
     return p;
   }
   // take over a buffer allocated elsewhere (null marks a failure)
@@ -124,7 +141,10 @@ class Scratch {
   int ok(const char* what) const;
 
  private:
+  static constexpr size_t kSmall = size_t(64) << 10, kChunk = size_t(1) << 20;
   std::vector<void*> bufs_;
+  char* chunk_ = nullptr;
+  size_t used_ = 0;
   bool failed_ = false;
 };
 
