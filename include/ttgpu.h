@@ -121,7 +121,8 @@ int ttg_contract(ttg_tensor A, ttg_tensor B, int npairs, const int* pa, const in
 int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_like,
                     const int* pairs3, int prec, ttg_heff* out);
 /* the same operator at another precision, on src's plans (output blocks and
- * task lists copied on the device, no QN bookkeeping) and src's tile shapes;
+ * task lists copied on the device, no QN bookkeeping) and src's tile shapes,
+ * applied by direct stage launches (no graph: it serves a few products);
  * src's L, W12 and R are read by both.  The FP64 twin serves the residual
  * correction of the reduced forms (tensor_davidson.cpp:255-276 computes its
  * products with the one FP64 operator) */

@@ -240,9 +240,11 @@ static int record(ttg_heff h, hipStream_t s) {
 }
 
 // intermediates, reduced-form operand copies (FP32 or split words), the measured tile
-// shapes when `tune`, and the captured apply graph of an operator whose three
-// plans exist; 1 with the error set on failure (the caller deletes h)
-static int heff_finish(ttg_heff h, bool tune) {
+// shapes when `tune`, and (when `capture`) the captured apply graph of an
+// operator whose three plans exist; 1 with the error set on failure (the
+// caller deletes h).  An operator without a graph launches its stages
+// directly on each apply.
+static int heff_finish(ttg_heff h, bool tune, bool capture) {
   const bool red = h->prec != TTG_FP64;
   const size_t es = red ? sizeof(float) : sizeof(double);
   h->T1 = dmalloc(std::max<size_t>(1, h->p[0]->C->total) * es);
@@ -282,6 +284,7 @@ static int heff_finish(ttg_heff h, bool tune) {
       if (tune_stage(h->p[st], st, io.a, io.b, io.c, io.f64, s)) return 1;
     }
   }
+  if (!capture) return 0;
   // capture the apply sequence once
   TTG_HIP(hipStreamSynchronize(s));
   TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -335,7 +338,7 @@ int ttg_heff_create(ttg_tensor L, ttg_tensor W12, ttg_tensor R, ttg_tensor x_lik
     set_error("heff: L*x*W12*R does not reproduce x's index sectors");
     return 1;
   }
-  if (remap_to_x(h->p[2], h->xm) || heff_finish(h, true)) {
+  if (remap_to_x(h->p[2], h->xm) || heff_finish(h, true, true)) {
     delete h;
     return 1;
   }
@@ -358,9 +361,10 @@ int ttg_heff_create_twin(ttg_heff src, int prec, ttg_heff* out) {
       return 1;
     }
   }
-  // the source's tile shapes (a twin serves a few products: timing
-  // candidate shapes for it would cost more than it saves)
-  if (heff_finish(h, false)) {
+  // the source's tile shapes and no graph (a twin serves a few products:
+  // timing candidate shapes or capturing and instantiating a graph for it
+  // would cost more than they save)
+  if (heff_finish(h, false, false)) {
     delete h;
     return 1;
   }
@@ -381,7 +385,8 @@ int ttg_heff_apply(ttg_heff h, ttg_tensor x, ttg_tensor y) {
   hipStream_t s = stream();
   const size_t bytes = h->xm->total * sizeof(double);
   TTG_HIP(hipMemcpyAsync(h->xbuf, x->d_data, bytes, hipMemcpyDeviceToDevice, s));
-  TTG_HIP(hipGraphLaunch(h->exec, s));
+  if (h->exec) TTG_HIP(hipGraphLaunch(h->exec, s));
+  else TTG_TRY(record(h, s));
   TTG_HIP(hipMemcpyAsync(y->d_data, h->ybuf, bytes, hipMemcpyDeviceToDevice, s));
   mark_written(y);
   return 0;
@@ -409,7 +414,8 @@ int ttg_heff_time(ttg_heff h, ttg_tensor x, ttg_tensor y, int n, double* ms) {
   TTG_HIP(hipEventRecord(e0, s));
   for (int i = 0; i < n; i++) {   // whole applies: x in, graph, y out
     TTG_HIP(hipMemcpyAsync(h->xbuf, x->d_data, bytes, hipMemcpyDeviceToDevice, s));
-    TTG_HIP(hipGraphLaunch(h->exec, s));
+    if (h->exec) TTG_HIP(hipGraphLaunch(h->exec, s));
+    else TTG_TRY(record(h, s));
     TTG_HIP(hipMemcpyAsync(y->d_data, h->ybuf, bytes, hipMemcpyDeviceToDevice, s));
   }
   TTG_HIP(hipEventRecord(e1, s));
