@@ -33,8 +33,14 @@ struct KeyArrays {
 };
 
 // ---- K1: per-block keys ------------------------------------------------
-__global__ void k_plan_keys(MetaView A, MetaView B, PairSpec ps, KeyArrays k) {
+// Block 0 also clears the plan header that the later planning kernels fill
+// (in place of a separate memset launch before this one).
+static_assert(sizeof(PlanHdr) % sizeof(int64_t) == 0, "plan header is whole 8-byte words");
+__global__ void k_plan_keys(MetaView A, MetaView B, PairSpec ps, KeyArrays k, PlanHdr* hdr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int w = threadIdx.x; w < (int)(sizeof(PlanHdr) / sizeof(int64_t)); w += blockDim.x)
+      ((int64_t*)hdr)[w] = 0;
   if (i < A.nblocks) {
     const int* qi = A.qi + (size_t)i * A.rank;
     int q0 = 0, q1 = 0;
@@ -516,8 +522,7 @@ __global__ void __launch_bounds__(1024) k_plan_scan(int n, const int64_t* size, 
 ttg_plan_s::~ttg_plan_s() {
   ttg::dfree(d_tasks);
   ttg::dfree(d_cblk);
-  ttg::dfree(d_tiles);
-  ttg::dfree(d_groups);
+  ttg::dfree(d_tiles);   // d_groups lies in the same allocation
   ttg::dfree(d_part);
   ttg::dfree(d_tuples);
 }
@@ -627,13 +632,16 @@ int build_tiles(ttg_plan_s& p, const std::vector<CBlock>& cb) {
   p.ntiles = (int)tiles.size();
   p.ngroups = (int)groups.size();
   p.nslots = nslots;
-  p.d_tiles = dalloc<Tile>(std::max<size_t>(1, tiles.size()));
-  p.d_groups = dalloc<SplitGroup>(std::max<size_t>(1, groups.size()));
+  // tiles and reduction groups share one allocation and one upload
+  const size_t tbytes = sizeof(Tile) * tiles.size(), goff = (tbytes + 255) & ~size_t(255);
+  const size_t gbytes = sizeof(SplitGroup) * groups.size();
+  p.d_tiles = (Tile*)dmalloc(goff + std::max<size_t>(gbytes, sizeof(SplitGroup)));
+  p.d_groups = p.d_tiles ? (SplitGroup*)((char*)p.d_tiles + goff) : nullptr;
   const size_t acc = p.prec == TTG_FP64 ? sizeof(double) : sizeof(float);
   p.d_part = nslots ? dmalloc((size_t)nslots * TM * TN * acc) : nullptr;
-  TTG_REQUIRE(p.d_tiles && p.d_groups && (p.d_part || !nslots), "plan: work list allocation failed");
-  TTG_TRY(h2d_vec(p.d_tiles, tiles));
-  TTG_TRY(h2d_vec(p.d_groups, groups));
+  TTG_REQUIRE(p.d_tiles && (p.d_part || !nslots), "plan: work list allocation failed");
+  const H2DPiece pc[2] = {{tiles.data(), 0, tbytes}, {groups.data(), goff, gbytes}};
+  TTG_TRY(h2d_pieces(p.d_tiles, pc, 2));
   return 0;
 }
 
@@ -645,7 +653,7 @@ int rebuild_tiles(ttg_plan_s& p, int cfg) {
     const D2HPiece pc{p.hcb.data(), p.d_cblk, sizeof(CBlock) * p.nblocksC};
     TTG_TRY(d2h_wait(&pc, 1));
   }
-  dfree(p.d_tiles); dfree(p.d_groups); dfree(p.d_part);
+  dfree(p.d_tiles); dfree(p.d_part);   // d_groups lies in d_tiles' allocation
   p.d_tiles = nullptr; p.d_groups = nullptr; p.d_part = nullptr;
   p.ngroups = 0;
   p.nslots = 0;
@@ -774,10 +782,9 @@ int plan_build(const MetaPtr& A, const MetaPtr& B, int np, const int* pa, const 
   TTG_TRY(sc.ok("plan"));
   PlanHdr* h_hdr = pinned_hdr();
   TTG_REQUIRE(h_hdr, "plan: pinned host header allocation failed");
-  TTG_HIP(hipMemsetAsync(d_hdr, 0, sizeof(PlanHdr), s));
 
   hipLaunchKernelGGL(k_plan_keys, dim3((nmax + 255) / 256), dim3(256), 0, s, A->view(), B->view(),
-                     ps, k);
+                     ps, k, d_hdr);
   SortJobs jobs;
   jobs.j[0] = {k.kUL, nullptr, k.sUL, nullptr, nA};
   jobs.j[1] = {k.kSA, k.vSA, k.sSA, k.svSA, nA};

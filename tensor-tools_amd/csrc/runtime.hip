@@ -178,6 +178,29 @@ int h2d(void* dst, const void* src, size_t bytes) {
   return 0;
 }
 
+int h2d_pieces(void* dst, const H2DPiece* pieces, int n) {
+  size_t total = 0;
+  for (int i = 0; i < n; i++) total = std::max(total, pieces[i].off + pieces[i].bytes);
+  if (!total) return 0;
+  TTG_TRY(ensure_init());
+  hipStream_t s = g_rt.stream;
+  if (total > kRingBytes / 4) {   // large: one direct copy per piece
+    for (int i = 0; i < n; i++) TTG_TRY(h2d((char*)dst + pieces[i].off, pieces[i].src, pieces[i].bytes));
+    return 0;
+  }
+  if (!g_ring.p) TTG_HIP(hipHostMalloc((void**)&g_ring.p, kRingBytes));
+  size_t off = (g_ring.off + 255) & ~size_t(255);
+  if (off + total > kRingBytes) {
+    TTG_HIP(hipStreamSynchronize(s));
+    off = 0;
+  }
+  for (int i = 0; i < n; i++)
+    if (pieces[i].bytes) memcpy(g_ring.p + off + pieces[i].off, pieces[i].src, pieces[i].bytes);
+  TTG_HIP(hipMemcpyAsync(dst, g_ring.p + off, total, hipMemcpyHostToDevice, s));
+  g_ring.off = off + total;
+  return 0;
+}
+
 int d2h_wait(const D2HPiece* pieces, int n) {
   TTG_TRY(ensure_init());
   hipStream_t s = g_rt.stream;

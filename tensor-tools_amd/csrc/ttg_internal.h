@@ -80,6 +80,15 @@ template <class T>
 int h2d_vec(T* dst, const std::vector<T>& v) {
   return h2d(dst, v.data(), sizeof(T) * v.size());
 }
+// host-to-device copy of several host ranges into one device buffer, at byte
+// offsets `off` (the bytes between pieces are left unspecified), as one copy
+// on the library stream; returns once the bytes are staged, like h2d
+struct H2DPiece {
+  const void* src;
+  size_t off;
+  size_t bytes;
+};
+int h2d_pieces(void* dst, const H2DPiece* pieces, int n);
 // device-to-host copies of several ranges into pinned staging, one wait for
 // the library stream, then into the destinations
 struct D2HPiece {

@@ -102,7 +102,7 @@ struct ttg_plan_s {
   ttg::GemmTask* d_tasks = nullptr;
   ttg::CBlock* d_cblk = nullptr;
   ttg::Tile* d_tiles = nullptr;
-  ttg::SplitGroup* d_groups = nullptr;   // split-K reduction groups
+  ttg::SplitGroup* d_groups = nullptr;   // split-K reduction groups (inside d_tiles' allocation)
   int ngroups = 0;
   void* d_part = nullptr;                // split-K partial tiles (accumulator type)
   int64_t nslots = 0;

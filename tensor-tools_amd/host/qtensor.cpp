@@ -287,6 +287,24 @@ void big_qtensor<double>::ensure_plan(const qtensor<double>& v) {
   if (g_heff_prec != TTG_FP64)   // FP64 twin on the same plans, for the correction products
     check(ttg_heff_create_twin(heff, TTG_FP64, &heff64), "heff_create (FP64 correction)");
   plan_prec = g_heff_prec;
+  // TTG_HEFF_LOG set (and not "0"): one stderr line per operator with its
+  // per-stage FLOPs, task counts, work shapes and work-item counts
+  static const bool log = [] {
+    const char* e = std::getenv("TTG_HEFF_LOG");
+    return e && *e && *e != '0';
+  }();
+  if (log) {
+    double f = 0, sf[3] = {0, 0, 0};
+    int64_t nt[3] = {0, 0, 0};
+    int cfg[3] = {0, 0, 0}, ntl[3] = {0, 0, 0}, rank = 0, nb = 0;
+    size_t total = 0;
+    if (ttg_heff_info(heff, &f, nt, sf) == 0 && ttg_heff_tiles(heff, cfg, ntl) == 0 &&
+        ttg_tensor_info(full.dev.get(), &rank, &nb, &total) == 0)
+      std::fprintf(stderr, "[heff] x %zu elems %d blocks, GF %.4f %.4f %.4f, tasks %lld %lld %lld, shape %d %d %d, items %d %d %d\n",
+                   total, nb, sf[0] * 1e-9, sf[1] * 1e-9, sf[2] * 1e-9,
+                   (long long)nt[0], (long long)nt[1], (long long)nt[2], cfg[0], cfg[1], cfg[2], ntl[0], ntl[1],
+                   ntl[2]);
+  }
 }
 
 // x in the operator's layout (the full legal block list of its indices)
